@@ -1,0 +1,273 @@
+"""Benchmark: noisy gates/s of the density-matrix update path on one MI355X.
+
+Workload (the headline case): 16 qubits (4^16 Pauli coefficients), 1000
+seeded gates -- U3 with probability 0.6 (angles U[0, 2pi)), else CX on
+(q, q +- d), d ~ Geometric(0.5) -- each followed by a depolarising channel
+p = 1e-3 on its qubits.  One step applies the whole 1000-gate program to the
+resident state.
+
+Paths measured:
+  plain_fp64   one kernel per noisy gate, FP64 storage (the baseline);
+  fused_fp32   gates fused into k<=4-qubit Pauli-transfer blocks (MFMA), FP32
+               storage, partition chosen by the learned cost model when a
+               trained model is available, else by the greedy rule (the
+               chosen method is reported in config.fusion).
+``value`` is the fused FP32 throughput; the plain FP64 figure, the speed-up
+and the accuracy of the FP32 result against FP64 are reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+os.environ.setdefault('OMP_NUM_THREADS', '1')  # the CPU baseline is a one-core numpy run
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, 'qiskit-aakash_amd')):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+from dmsim import fusion, native, workloads  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {'fp32': 157.3, 'fp64': 78.6}
+METRIC = 'noisy gates/sec at 16q: learned-fused FP32 vs plain FP64; % HBM peak'
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def zero_state(st, n):
+    st.init_product(np.tile([1.0, 0.0, 0.0, 1.0], n), 0.5 ** n)
+
+
+def time_program(prog, n, precision, steps, warmup):
+    """Wall time of `steps` program applications (after `warmup`), with HIP
+    event timing of every launch on the handle's stream."""
+    st = native.DeviceState(n, precision)
+    zero_state(st, n)
+    for _ in range(warmup):
+        prog.run(st)
+    st.synchronize()
+    st.profile(True)
+    st.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        prog.run(st)
+    st.synchronize()
+    t1 = time.perf_counter()
+    fams = {f: st.profile_read(f) for f in native.FAMILIES}
+    st.profile(False)
+    st.close()
+    return t1 - t0, fams
+
+
+def kernel_roofline(fams, n, precision):
+    """Roofline of the kernel family with the largest total time."""
+    s = 8 if precision == 'fp64' else 4
+    N = 4 ** n
+    dom = max(fams, key=lambda f: fams[f][1])
+    launches, ms = fams[dom]
+    avg_s = ms / 1e3 / max(1, launches)
+    if dom in ('block3', 'block4'):
+        k = 3 if dom == 'block3' else 4
+        flops = 2.0 * 4 ** k * N
+        byts = 2.0 * N * s
+        intensity = flops / byts
+        ridge = MFMA_PEAK_TFLOPS[precision] * 1e12 / (HBM_PEAK_GBS * 1e9)
+        if intensity >= ridge:
+            return dom, {'bound': 'mfma', 'achieved': flops / avg_s / 1e12, 'peak': MFMA_PEAK_TFLOPS[precision],
+                         'unit': 'TFLOP/s', 'frac': flops / avg_s / 1e12 / MFMA_PEAK_TFLOPS[precision],
+                         'traffic': None}, avg_s, byts, flops
+        ach = byts / avg_s / 1e9
+        return dom, {'bound': 'hbm', 'achieved': ach, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': ach / HBM_PEAK_GBS, 'traffic': None}, avg_s, byts, flops
+    per = {'1q': 2 * 0.75, 'cx': 2 * 15 / 16, '2q': 2 * 15 / 16, 'memory_noise': 2.0, 'diag': 2.0}.get(dom, 2.0)
+    byts = per * N * s
+    ach = byts / avg_s / 1e9
+    return dom, {'bound': 'hbm', 'achieved': ach, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                 'frac': ach / HBM_PEAK_GBS, 'traffic': None}, avg_s, byts, 0.0
+
+
+def streaming_fraction(fams, n, precision, counts):
+    """Sum of compulsory bytes of the plain streaming kernels / their time / peak."""
+    s = 8 if precision == 'fp64' else 4
+    N = 4 ** n
+    tot_b = tot_t = 0.0
+    for fam, per in (('1q', 1.5), ('cx', 2 * 15 / 16), ('2q', 2 * 15 / 16)):
+        launches, ms = fams[fam]
+        if launches:
+            tot_b += per * N * s * launches
+            tot_t += ms / 1e3
+    return (tot_b / tot_t / 1e9 / HBM_PEAK_GBS) if tot_t else None
+
+
+def accuracy(prog_fp32, prog_fp64, n):
+    """One application of each program from |0..0>: FP32 result vs FP64."""
+    a = native.DeviceState(n, 'fp64')
+    zero_state(a, n)
+    prog_fp64.run(a)
+    ref = a.download()
+    a.close()
+    b = native.DeviceState(n, 'fp32')
+    zero_state(b, n)
+    prog_fp32.run(b)
+    tr, pur = b.trace_purity()
+    marg = [b.probabilities([q], 'Z')[0] for q in range(n)]
+    fid = b.overlap(ref)
+    got = b.download()
+    b.close()
+    err = 0.0
+    chunk = 1 << 26
+    for i in range(0, ref.size, chunk):
+        err = max(err, float(np.max(np.abs(ref[i:i + chunk] - got[i:i + chunk]))))
+    ref_pur = float(2.0 ** n * np.dot(ref, ref))
+    ref_marg = []
+    for q in range(n):
+        # P(q=0) = (1 + 2^n a_{Z on q})/2
+        ref_marg.append(0.5 * (1.0 + ref[3 * 4 ** (n - 1 - q)] * 2.0 ** n))
+    return {'max_abs_coeff_err': err, 'max_abs_coeff_err_x2n': err * 2.0 ** n,
+            'fidelity_fp32_vs_fp64': fid / ref_pur, 'trace_fp32': tr, 'purity_fp32': pur, 'purity_fp64': ref_pur,
+            'max_marginal_err': float(np.max(np.abs(np.array(marg) - np.array(ref_marg)))),
+            'identity_coeff_exact': bool(got[0] == 2.0 ** -n)}
+
+
+def cpu_baseline(n_target, G, seed, budget_s=20.0, n_sample=13):
+    """The oracle (FP64 numpy restatement of the reference, one core) on the
+    same workload generator at `n_sample` qubits until `budget_s` elapses;
+    gates/s scaled to `n_target` qubits by 4^-(n_target - n_sample) (the work
+    per gate is proportional to 4^n)."""
+    from oracle import refsim
+    n = n_sample
+    gates = workloads.local_u3_cx(n, G, seed)
+    st = refsim.initial_state(n)
+    done = 0
+    t0 = time.perf_counter()
+    for name, qs, prm in gates:
+        if name == 'u3':
+            q = qs[0]
+            s3 = st.reshape(4 ** q, 4, 4 ** (n - q - 1))
+            for axis, ang in refsim.gate_rotations('u3', prm):
+                s3 = refsim.rotate_components(s3, axis, ang, [1.0, 0.0])
+            st = refsim.apply_gate_depolarizing(s3.reshape(n * [4]), n, [q], 1e-3)
+        else:
+            st = refsim.apply_cx(st, qs[0], qs[1], [1.0, 0.0], n)
+            st = refsim.apply_gate_depolarizing(st, n, list(qs), 1e-3)
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    rate = done / el
+    return {'value': rate / 4 ** (n_target - n), 'unit': 'gates/s', 'cores': 1, 'kind': 'port',
+            'sample': '%d gates of the same generator at %d qubits (FP64 oracle, numpy, 1 thread) in %.1f s = '
+                      '%.3f gates/s; scaled by 4^-%d to %d qubits' % (done, n, el, rate, n_target - n, n_target)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=2)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--qubits', type=int, default=16)
+    ap.add_argument('--gates', type=int, default=1000)
+    ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-accuracy', action='store_true')
+    ap.add_argument('--fusion', default='auto', choices=['auto', 'greedy', 'learned'])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py runs on one GPU: the path has no multi-GPU form in this build (--gpus %d refused)'
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    n, G = args.qubits, args.gates
+    gates = workloads.local_u3_cx(n, G, args.seed)
+    noisy = fusion.noisy_gates(gates, 1e-3)
+
+    method = args.fusion
+    cost = None
+    if method in ('auto', 'learned'):
+        try:
+            from dmsim import costmodel
+            cost = costmodel.default_model()
+            method = 'learned'
+        except Exception as e:  # no trained model shipped
+            if args.fusion == 'learned':
+                raise
+            log('no learned cost model (%s); greedy fusion' % e)
+            method = 'greedy'
+    t0 = time.perf_counter()
+    fused, blocks = fusion.fused_program(noisy, n, 4, method, cost)
+    plan_s = time.perf_counter() - t0
+    plain = fusion.plain_program(noisy, n)
+    ks = [len(q) if len(m) > 1 else 0 for q, m in blocks]
+    log('plan: %s, %d passes for %d gates (%.2f s host); k histogram %s'
+        % (method, len(fused), G, plan_s, np.bincount(ks, minlength=5).tolist()))
+
+    log('timing plain fp64 ...')
+    t_plain, f_plain = time_program(plain, n, 'fp64', args.steps, args.warmup)
+    log('  %.3f s for %d steps' % (t_plain, args.steps))
+    log('timing fused fp32 ...')
+    t_fused, f_fused = time_program(fused, n, 'fp32', args.steps, args.warmup)
+    log('  %.3f s for %d steps' % (t_fused, args.steps))
+    log('timing plain fp32 / fused fp64 ...')
+    t_plain32, f_plain32 = time_program(plain, n, 'fp32', args.steps, args.warmup)
+    t_fused64, _ = time_program(fused, n, 'fp64', args.steps, args.warmup)
+
+    gps = lambda t: G * args.steps / t
+    value = gps(t_fused)
+    dom, roof, avg_s, byts, flops = kernel_roofline(f_fused, n, 'fp32')
+    pdom, proof, _, _, _ = kernel_roofline(f_plain, n, 'fp64')
+    acc = None if args.no_accuracy else accuracy(fused, plain, n)
+    if acc:
+        log('accuracy: %s' % json.dumps(acc))
+    cpu = None if args.no_cpu_baseline else cpu_baseline(n, G, args.seed)
+    if cpu:
+        log('cpu baseline: %s' % cpu['sample'])
+
+    out = {
+        'metric': METRIC,
+        'value': value,
+        'unit': 'gates/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': t_fused / args.steps * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic',
+        'config': {
+            'workload': '%dq local U3/CX 60/40 (CX distance ~ Geometric(0.5)), %d gates, depolarizing 1e-3 per '
+                        'gate, seed %d' % (n, G, args.seed),
+            'path': 'fused fp32 (%s partition, %d passes)' % (method, len(fused)),
+            'fusion': method,
+            'plain_fp64_gates_per_s': gps(t_plain),
+            'plain_fp32_gates_per_s': gps(t_plain32),
+            'fused_fp64_gates_per_s': gps(t_fused64),
+            'speedup_fused_fp32_vs_plain_fp64': value / gps(t_plain),
+            'plain_fp64_dominant_kernel': pdom,
+            'plain_fp64_roofline': proof,
+            'plain_fp64_streaming_hbm_frac': streaming_fraction(f_plain, n, 'fp64', None),
+            'plain_fp32_streaming_hbm_frac': streaming_fraction(f_plain32, n, 'fp32', None),
+            'fused_fp32_kernel_ms': {k: v[1] / max(1, v[0]) for k, v in f_fused.items() if v[0]},
+            'fused_fp32_launches': {k: v[0] for k, v in f_fused.items() if v[0]},
+            'plan_host_s': plan_s,
+            'accuracy_fp32_vs_fp64': acc,
+        },
+        'roofline': dict(roof, kernel=dom, avg_launch_ms=avg_s * 1e3, algorithmic_bytes=byts,
+                         algorithmic_flops=flops),
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

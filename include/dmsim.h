@@ -1,0 +1,174 @@
+/*
+ * dmsim.h -- C ABI of the MI355X Pauli-basis density-matrix engine.
+ *
+ * The state of an n-qubit register is the vector of its 4^n real Pauli
+ * coefficients a_{i_0 ... i_{n-1}} (i in {0,1,2,3} = {I,X,Y,Z}), resident in
+ * HBM.  Qubit q is the digit of weight 4^(n-1-q) of the flat index (qubit 0 is
+ * the slowest digit), the layout of the reference backend
+ * (qiskit/providers/basicaer/dm_simulator.py:348-349).  Storage is FP64 or
+ * FP32; every reduction accumulates in FP64.
+ *
+ * Every entry point returns 0 on success or a negative DM_ERR_* code;
+ * dm_last_error() gives the message of the calling thread's last failure.
+ * A handle owns one HIP stream and its device buffers; calls on one handle
+ * must not overlap (separate handles may run on separate threads).  Calls
+ * that return data to the host synchronise the handle's stream; the others
+ * only enqueue work.
+ *
+ * Each declaration names the reference function it replaces.
+ */
+#ifndef DMSIM_H
+#define DMSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dm_state dm_state;
+
+enum { DM_FP64 = 0, DM_FP32 = 1 };
+
+enum {
+  DM_OK = 0,
+  DM_ERR_ARG = -1,     /* bad argument (qubit index, size, null pointer) */
+  DM_ERR_HIP = -2,     /* HIP runtime failure (incl. out of device memory) */
+  DM_ERR_STATE = -3    /* handle in the wrong state for the call */
+};
+
+/* Message of the calling thread's last failure (always NUL-terminated). */
+int dm_last_error(char *buf, size_t len);
+
+/* Library / device description: number of visible devices, compute units and
+ * HBM bytes of device `device`.  No counterpart in the reference. */
+int dm_device_info(int device, int *n_devices, int *cu_count, uint64_t *hbm_bytes);
+
+/* Allocate an n-qubit state (n in [1, 20]) on the current HIP device.
+ * Replaces the numpy state owned by DmSimulatorPy (dm_simulator.py:143-175). */
+int dm_create(int n_qubits, int dtype, dm_state **out);
+int dm_destroy(dm_state *h);
+int dm_num_qubits(const dm_state *h, int *n);
+int dm_dtype(const dm_state *h, int *dtype);
+/* Device address of the coefficient vector (for timing harnesses). */
+int dm_device_ptr(const dm_state *h, void **ptr);
+
+/* Product state: a = (x)_q v_q with v_q = per_qubit4[4q .. 4q+3] (qubit 0
+ * slowest), times `scale`.  Replaces the np.kron construction of
+ * _initialize_densitymatrix (dm_simulator.py:284-349). */
+int dm_init_product(dm_state *h, const double *per_qubit4, double scale);
+
+/* Host <-> device copies of all 4^n coefficients as FP64 (converted in the
+ * FP32 form).  dm_upload replaces the stored-state load (dm_simulator.py:
+ * 337-343); dm_download the coefficient export of _get_densitymatrix
+ * (:1257-1269).  dm_download_chop zeroes |a| < thr on the device first (the
+ * reference chops the live state, :1262-1263). */
+int dm_upload(dm_state *h, const double *host);
+int dm_download(dm_state *h, double *host);
+int dm_download_chop(dm_state *h, double thr, double *host);
+/* Read k single coefficients by flat index. */
+int dm_read_coeffs(dm_state *h, const uint64_t *index, int k, double *out);
+
+/* General single-qubit Pauli transfer matrix on qubit q: rows X,Y,Z of the
+ * 4x4 PTM, row-major (R[4*r + c] is the coefficient of input component c in
+ * output component r+1); the I row is the identity (trace preserving).
+ * Replaces rot_gate_dm_matrix via _add_unitary_single (basicaertools.py:
+ * 93-126, dm_simulator.py:367-384), _add_qasm_reset (:810-823),
+ * _add_qasm_measure_{X,Y,Z,N} (:574-704). */
+int dm_apply_1q(dm_state *h, int q, const double R[12]);
+
+/* CX(control, target) as a signed permutation of the 16 two-qubit Pauli
+ * pairs, each output pair (c,t) (index 4c+t, control component first)
+ * multiplied by scale[4c+t] (pass NULL for a noiseless CX; a per-gate
+ * depolarising channel is a scale).  Replaces cx_gate_dm_matrix with
+ * tsp_model_error = [1, 0] (basicaertools.py:310-392). */
+int dm_apply_cx(dm_state *h, int control, int target, const double *scale16);
+
+/* Dense 16x16 two-qubit PTM on (qa, qb): R[16*r + c], pair index 4*i_qa + i_qb.
+ * Used for the transition-selective-pulse CX error (basicaertools.py:
+ * 328-388) and fused two-qubit blocks. */
+int dm_apply_2q(dm_state *h, int qa, int qb, const double R[256]);
+
+/* Memory noise on every qubit: x,y <- sqrt(g) f (x,y); z <- g z + (1-g)(2p-1) a_I.
+ * Replaces _add_decoherence_and_amp_decay (dm_simulator.py:397-425). */
+int dm_memory_noise(dm_state *h, double f, double p, double g);
+
+/* Per-qubit diagonal scaling: a_i *= prod_q tab[4q + i_q] (collapse onto a
+ * Pauli string, multi-qubit depolarising).  Replaces the slice scaling of
+ * _pauli_string_expectation (dm_simulator.py:540-572). */
+int dm_apply_diag(dm_state *h, const double *tab4n);
+
+/* Fused block: a k-qubit Pauli transfer matrix S (4^k x 4^k, FP64 host,
+ * row-major, component index sum_j i_{qubits[j]} 4^j) applied to the state
+ * (MFMA for k = 3, 4).  No reference counterpart (the reference applies gates
+ * one by one). */
+int dm_apply_block(dm_state *h, const int *qubits, int k, const double *S);
+
+/* Outcome probabilities of measuring qubits[0..m-1] (ascending) along basis
+ * 'X','Y','Z' (or 'N' with unit vector nvec) with polarisation retention err,
+ * marginalised over the other qubits: out[b], b's most significant bit is
+ * qubits[0].  FP64 gather + Walsh-Hadamard contraction on the device.
+ * Replaces _add_ensemble_measure / _add_partial_measure's probabilities
+ * (dm_simulator.py:427-538). */
+int dm_probabilities(dm_state *h, const int *qubits, int m, char basis,
+                     const double *nvec, double err, double *out);
+
+/* tr = 2^n a_0; purity = 2^n sum a^2 (FP64). */
+int dm_trace_purity(dm_state *h, double *tr, double *purity);
+
+/* 2^n <a, b> with b given on the host (FP64).  Replaces _state_overlap
+ * (dm_simulator.py:1277-1282). */
+int dm_overlap(dm_state *h, const double *other_host, double *out);
+
+/* 2^n x 2^n complex128 density matrix (row-major, interleaved re/im), qubit 0
+ * the most significant bit of row and column.  Replaces _compute_densitymatrix
+ * (dm_simulator.py:1198-1255). */
+int dm_to_dense(dm_state *h, double *out_c128);
+
+/* A program: a list of operations executed back to back on the handle's
+ * stream with one upload of all block matrices and no host synchronisation.
+ * Replaces the per-level dispatch loop of run_experiment (dm_simulator.py:
+ * 1005-1177) for the unitary / channel part of a circuit.
+ * Parameters of op i start at params[ops[i].param]:
+ *   DM_OP_1Q            qubits[0], 12 values (rows X,Y,Z of the 4x4 PTM)
+ *   DM_OP_CX            qubits[0]=control, qubits[1]=target, no values
+ *   DM_OP_CX_SCALED     as DM_OP_CX, 16 per-pair scales
+ *   DM_OP_2Q            qubits[0..1], 256 values (dense 16x16 PTM)
+ *   DM_OP_BLOCK         k = 3 or 4 qubits, 4^(2k) values (block PTM)
+ *   DM_OP_MEMORY_NOISE  3 values f, p, g
+ *   DM_OP_DIAG          4n values (per-qubit diagonal tables)            */
+enum {
+  DM_OP_1Q = 1,
+  DM_OP_CX = 2,
+  DM_OP_CX_SCALED = 3,
+  DM_OP_2Q = 4,
+  DM_OP_BLOCK = 5,
+  DM_OP_MEMORY_NOISE = 6,
+  DM_OP_DIAG = 7
+};
+
+typedef struct dm_op {
+  int32_t kind;
+  int32_t k;          /* number of entries of qubits[] in use */
+  int32_t qubits[4];
+  int64_t param;      /* offset of the op's values in params[] */
+} dm_op;
+
+int dm_run_program(dm_state *h, const dm_op *ops, int nops, const double *params, int64_t nparams);
+
+/* Stream control and timing (no reference counterpart). */
+int dm_synchronize(dm_state *h);
+/* Accumulated GPU time per kernel family since the last reset, measured with
+ * HIP events around each launch when profiling is on. family: 0 = 1q, 1 = cx,
+ * 2 = 2q dense, 3 = memory noise, 4 = diag, 5 = block k=3, 6 = reductions,
+ * 7 = init/copies, 8 = block k=4. */
+int dm_profile(dm_state *h, int enable);
+int dm_profile_read(dm_state *h, int family, uint64_t *launches, double *ms);
+int dm_profile_reset(dm_state *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DMSIM_H */

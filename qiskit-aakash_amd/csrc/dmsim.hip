@@ -1,0 +1,1623 @@
+// dmsim.hip -- CDNA4 (gfx950) kernels and the C ABI of the Pauli-basis
+// density-matrix engine.  See include/dmsim.h for the contract.
+//
+// Layout: a flat vector of N = 4^n coefficients; the 2-bit digit at
+// "position" p = n-1-q of the flat index is the Pauli component of qubit q
+// (stride 4^p).  All kernels update the vector in place; every element is
+// read and written by exactly one thread (or one workgroup for tiles), so
+// in-place updates are race free.
+
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "dmsim.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxQubits = 20;
+constexpr int kFamilies = 10;  // profiling slots, see dm_profile_read
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIP_TRY(expr)                                                            \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess)                                                        \
+      return fail(DM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));    \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Small vector helpers
+// ---------------------------------------------------------------------------
+
+template <typename T, int W>
+struct alignas(W * sizeof(T)) Pack {
+  T v[W];
+};
+
+template <typename P, typename T>
+__device__ __forceinline__ P ldp(const T *p) {
+  return *reinterpret_cast<const P *>(p);
+}
+template <typename P, typename T>
+__device__ __forceinline__ void stp(T *p, const P &x) {
+  *reinterpret_cast<P *>(p) = x;
+}
+
+// Insert a zero 2-bit digit at each of the ascending positions pos[0..m-1].
+__device__ __forceinline__ uint64_t insert_digits(uint64_t r, const int *pos, int m) {
+  for (int j = 0; j < m; ++j) {
+    const int s = 2 * pos[j];
+    const uint64_t low = r & ((1ull << s) - 1ull);
+    r = ((r >> s) << (s + 2)) | low;
+  }
+  return r;
+}
+
+__host__ __device__ __forceinline__ uint64_t pow4(int p) { return 1ull << (2 * p); }
+
+// ---------------------------------------------------------------------------
+// K1: single-qubit PTM (rows X,Y,Z), any stride.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct Rows3 {
+  T r[3][4];
+};
+
+template <typename T>
+Rows3<T> make_rows3(const double *R) {
+  Rows3<T> m;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 4; ++j) m.r[i][j] = (T)R[4 * i + j];
+  return m;
+}
+
+// Stride S = 4^p >= W: each unit covers W consecutive low indices of one
+// (hi) slab; the four components are four coalesced W-vectors.
+template <typename T, bool USE_I>
+__global__ __launch_bounds__(kBlock) void k_1q_wide(T *__restrict__ a, uint64_t units, int sh,
+                                                    uint64_t S, Rows3<T> m) {
+  constexpr int W = 16 / sizeof(T);
+  using P = Pack<T, W>;
+  const uint64_t mask = (1ull << sh) - 1ull;
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < units; u += step) {
+    T *b = a + ((u >> sh) * 4ull) * S + (u & mask) * W;
+    P x0;
+    if (USE_I) x0 = ldp<P>(b);
+    const P x1 = ldp<P>(b + S), x2 = ldp<P>(b + 2 * S), x3 = ldp<P>(b + 3 * S);
+    P y1, y2, y3;
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+      T s1 = m.r[0][1] * x1.v[e] + m.r[0][2] * x2.v[e] + m.r[0][3] * x3.v[e];
+      T s2 = m.r[1][1] * x1.v[e] + m.r[1][2] * x2.v[e] + m.r[1][3] * x3.v[e];
+      T s3 = m.r[2][1] * x1.v[e] + m.r[2][2] * x2.v[e] + m.r[2][3] * x3.v[e];
+      if (USE_I) {
+        s1 += m.r[0][0] * x0.v[e];
+        s2 += m.r[1][0] * x0.v[e];
+        s3 += m.r[2][0] * x0.v[e];
+      }
+      y1.v[e] = s1;
+      y2.v[e] = s2;
+      y3.v[e] = s3;
+    }
+    stp(b + S, y1);
+    stp(b + 2 * S, y2);
+    stp(b + 3 * S, y3);
+  }
+}
+
+// Stride 1 (the last qubit): the four components of a group are adjacent.
+template <typename T, bool USE_I>
+__global__ __launch_bounds__(kBlock) void k_1q_p0(T *__restrict__ a, uint64_t groups, Rows3<T> m) {
+  using P = Pack<T, 4>;
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < groups; g += step) {
+    P x = ldp<P>(a + 4 * g);
+    P y;
+    y.v[0] = x.v[0];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      T s = m.r[r][1] * x.v[1] + m.r[r][2] * x.v[2] + m.r[r][3] * x.v[3];
+      if (USE_I) s += m.r[r][0] * x.v[0];
+      y.v[r + 1] = s;
+    }
+    stp(a + 4 * g, y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K2: CX as a signed permutation of the 16 (control, target) pairs, with an
+// optional per-pair scale.  Pairs are addressed in position order (hi, lo);
+// CTRL_HI says whether the control qubit sits at the higher position.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct Scale16 {
+  T s[16];
+};
+
+// Output pair (c,t) <- sign * input pair src(c,t), for the 12 moving pairs.
+// (0,2)<-(3,2) (3,2)<-(0,2) (0,3)<-(3,3) (3,3)<-(0,3) (1,0)<-(1,1) (1,1)<-(1,0)
+// (1,2)<-(2,3) (1,3)<--(2,2) (2,0)<-(2,1) (2,1)<-(2,0) (2,2)<--(1,3) (2,3)<-(1,2)
+template <typename V, typename T, bool CTRL_HI, bool SCALED>
+__device__ __forceinline__ void cx_map(V (&x)[4][4], const Scale16<T> &sc) {
+  auto X = [&](int c, int t) -> V & { return CTRL_HI ? x[c][t] : x[t][c]; };
+  auto S = [&](int c, int t) -> T { return SCALED ? sc.s[4 * c + t] : T(1); };
+  V t02 = X(0, 2), t32 = X(3, 2), t03 = X(0, 3), t33 = X(3, 3);
+  V t10 = X(1, 0), t11 = X(1, 1), t12 = X(1, 2), t13 = X(1, 3);
+  V t20 = X(2, 0), t21 = X(2, 1), t22 = X(2, 2), t23 = X(2, 3);
+  X(0, 2) = t32 * S(0, 2);
+  X(3, 2) = t02 * S(3, 2);
+  X(0, 3) = t33 * S(0, 3);
+  X(3, 3) = t03 * S(3, 3);
+  X(1, 0) = t11 * S(1, 0);
+  X(1, 1) = t10 * S(1, 1);
+  X(1, 2) = t23 * S(1, 2);
+  X(1, 3) = -t22 * S(1, 3);
+  X(2, 0) = t21 * S(2, 0);
+  X(2, 1) = t20 * S(2, 1);
+  X(2, 2) = -t13 * S(2, 2);
+  X(2, 3) = t12 * S(2, 3);
+  if (SCALED) {
+    X(0, 1) = X(0, 1) * S(0, 1);
+    X(3, 0) = X(3, 0) * S(3, 0);
+    X(3, 1) = X(3, 1) * S(3, 1);
+  }
+}
+
+// Whether pair (i_hi, i_lo) in position order is written by the CX map.
+__host__ __device__ constexpr bool cx_writes(bool ctrl_hi, bool scaled, int ih, int il) {
+  return (ih == 0 && il == 0) ? false
+         : scaled             ? true
+         : ctrl_hi            ? !((ih == 0 && il == 1) || (ih == 3 && il == 0) || (ih == 3 && il == 1))
+                              : !((il == 0 && ih == 1) || (il == 3 && ih == 0) || (il == 3 && ih == 1));
+}
+
+// Low position >= 1 (stride >= W): units of W consecutive low indices.
+template <typename T, bool CTRL_HI, bool SCALED>
+__global__ __launch_bounds__(kBlock) void k_cx_wide(T *__restrict__ a, uint64_t units, int sh_lo,
+                                                    int sh_mid, uint64_t Sh, uint64_t Sl,
+                                                    Scale16<T> sc) {
+  constexpr int W = 16 / sizeof(T);
+  using P = Pack<T, W>;
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < units; u += step) {
+    const uint64_t lo = u & ((1ull << sh_lo) - 1ull);
+    const uint64_t rest = u >> sh_lo;
+    const uint64_t mid = rest & ((1ull << sh_mid) - 1ull);
+    const uint64_t hi = rest >> sh_mid;
+    T *b = a + hi * 4ull * Sh + mid * 4ull * Sl + lo * W;
+    P x[4][4];
+#pragma unroll
+    for (int ih = 0; ih < 4; ++ih)
+#pragma unroll
+      for (int il = 0; il < 4; ++il)
+        if (cx_writes(CTRL_HI, SCALED, ih, il)) x[ih][il] = ldp<P>(b + ih * Sh + il * Sl);
+    // per-lane permutation
+    P y[4][4];
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+      T v[4][4];
+#pragma unroll
+      for (int ih = 0; ih < 4; ++ih)
+#pragma unroll
+        for (int il = 0; il < 4; ++il) v[ih][il] = cx_writes(CTRL_HI, SCALED, ih, il) ? x[ih][il].v[e] : T(0);
+      cx_map<T, T, CTRL_HI, SCALED>(v, sc);
+#pragma unroll
+      for (int ih = 0; ih < 4; ++ih)
+#pragma unroll
+        for (int il = 0; il < 4; ++il) y[ih][il].v[e] = v[ih][il];
+    }
+#pragma unroll
+    for (int ih = 0; ih < 4; ++ih)
+#pragma unroll
+      for (int il = 0; il < 4; ++il)
+        if (cx_writes(CTRL_HI, SCALED, ih, il)) stp(b + ih * Sh + il * Sl, y[ih][il]);
+  }
+}
+
+// Low position 0: each of the four high components is a contiguous quad.
+template <typename T, bool CTRL_HI, bool SCALED>
+__global__ __launch_bounds__(kBlock) void k_cx_p0(T *__restrict__ a, uint64_t groups, int sh_mid,
+                                                  uint64_t Sh, Scale16<T> sc) {
+  using P = Pack<T, 4>;
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < groups; g += step) {
+    const uint64_t mid = g & ((1ull << sh_mid) - 1ull);
+    const uint64_t hi = g >> sh_mid;
+    T *b = a + hi * 4ull * Sh + mid * 4ull;
+    T v[4][4];
+#pragma unroll
+    for (int ih = 0; ih < 4; ++ih) {
+      P q = ldp<P>(b + ih * Sh);
+#pragma unroll
+      for (int il = 0; il < 4; ++il) v[ih][il] = q.v[il];
+    }
+    cx_map<T, T, CTRL_HI, SCALED>(v, sc);
+#pragma unroll
+    for (int ih = 0; ih < 4; ++ih) {
+      bool any = false;
+#pragma unroll
+      for (int il = 0; il < 4; ++il) any = any || cx_writes(CTRL_HI, SCALED, ih, il);
+      if (any) {
+        P q;
+#pragma unroll
+        for (int il = 0; il < 4; ++il) q.v[il] = v[ih][il];
+        stp(b + ih * Sh, q);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K2': dense two-qubit PTM (16x16), pairs in position order (hi, lo).
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct Mat16 {
+  T m[16][16];
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_2q_dense(T *__restrict__ a, uint64_t groups, int sh_lo,
+                                                     int sh_mid, uint64_t Sh, uint64_t Sl,
+                                                     Mat16<T> R, uint32_t rowmask) {
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < groups; u += step) {
+    const uint64_t lo = u & ((1ull << sh_lo) - 1ull);
+    const uint64_t rest = u >> sh_lo;
+    const uint64_t mid = rest & ((1ull << sh_mid) - 1ull);
+    const uint64_t hi = rest >> sh_mid;
+    T *b = a + hi * 4ull * Sh + mid * 4ull * Sl + lo;
+    T x[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) x[j] = b[(j >> 2) * Sh + (j & 3) * Sl];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      if (rowmask & (1u << r)) {
+        T s = T(0);
+#pragma unroll
+        for (int c = 0; c < 16; ++c) s += R.m[r][c] * x[c];
+        b[(r >> 2) * Sh + (r & 3) * Sl] = s;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K3: register tile of M qubits (positions pos[0..M-1], ascending): each
+// thread owns the 4^M coefficients that differ only on those qubits and
+// applies one 4x4 map per tile qubit.  Used for memory noise with a thermal
+// feed (x,y <- h x,h y; z <- g z + d a_I) where the update is not diagonal.
+// ---------------------------------------------------------------------------
+
+template <typename T, int M>
+struct TileMaps {
+  int pos[M];
+  T A[M][4][4];
+};
+
+template <typename T, int M>
+__global__ __launch_bounds__(kBlock) void k_tile_maps(T *__restrict__ a, uint64_t units,
+                                                      TileMaps<T, M> tm) {
+  constexpr int E = 1 << (2 * M);
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  uint64_t S[M];
+#pragma unroll
+  for (int j = 0; j < M; ++j) S[j] = pow4(tm.pos[j]);
+  for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < units; u += step) {
+    const uint64_t base = insert_digits(u, tm.pos, M);
+    T x[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      uint64_t off = 0;
+#pragma unroll
+      for (int j = 0; j < M; ++j) off += (uint64_t)((e >> (2 * j)) & 3) * S[j];
+      x[e] = a[base + off];
+    }
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        if (((e >> (2 * j)) & 3) != 0) continue;
+        const int st = 1 << (2 * j);
+        T v0 = x[e], v1 = x[e + st], v2 = x[e + 2 * st], v3 = x[e + 3 * st];
+        x[e + st] = tm.A[j][1][0] * v0 + tm.A[j][1][1] * v1 + tm.A[j][1][2] * v2 + tm.A[j][1][3] * v3;
+        x[e + 2 * st] = tm.A[j][2][0] * v0 + tm.A[j][2][1] * v1 + tm.A[j][2][2] * v2 + tm.A[j][2][3] * v3;
+        x[e + 3 * st] = tm.A[j][3][0] * v0 + tm.A[j][3][1] * v1 + tm.A[j][3][2] * v2 + tm.A[j][3][3] * v3;
+      }
+    }
+#pragma unroll
+    for (int e = 1; e < E; ++e) {
+      uint64_t off = 0;
+#pragma unroll
+      for (int j = 0; j < M; ++j) off += (uint64_t)((e >> (2 * j)) & 3) * S[j];
+      a[base + off] = x[e];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K4: diagonal scalings.
+//   k_diag_tab:  a_i *= prod_p tab[p][digit_p(i)]       (per-qubit tables)
+//   k_diag_pow:  a_i *= hpow[#X/Y digits] * gpow[#Z digits] (same map on all)
+//   k_product:   a_i  = scale * prod_p vec[p][digit_p(i)]  (product states)
+// ---------------------------------------------------------------------------
+
+struct Tab4 {
+  double t[kMaxQubits * 4];
+};
+
+template <typename T, bool WRITE_ONLY>
+__global__ __launch_bounds__(kBlock) void k_diag_tab(T *__restrict__ a, uint64_t units, int n, Tab4 tab,
+                                                     double scale) {
+  constexpr int W = 16 / sizeof(T);
+  using P = Pack<T, W>;
+  __shared__ double s_tab[kMaxQubits * 4];
+  for (int i = threadIdx.x; i < 4 * n; i += kBlock) s_tab[i] = tab.t[i];
+  __syncthreads();
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < units; u += step) {
+    P x;
+    if (!WRITE_ONLY) x = ldp<P>(a + u * W);
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+      const uint64_t i = u * W + e;
+      double f = scale;
+      for (int p = 0; p < n; ++p) f *= s_tab[4 * p + ((i >> (2 * p)) & 3)];
+      x.v[e] = WRITE_ONLY ? (T)f : (T)(x.v[e] * f);
+    }
+    stp(a + u * W, x);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_diag_pow(T *__restrict__ a, uint64_t units, int n,
+                                                     double h, double g) {
+  constexpr int W = 16 / sizeof(T);
+  using P = Pack<T, W>;
+  __shared__ T s_h[kMaxQubits + 1], s_g[kMaxQubits + 1];
+  if (threadIdx.x == 0) {
+    double ph = 1.0, pg = 1.0;
+    for (int k = 0; k <= n; ++k) {
+      s_h[k] = (T)ph;
+      s_g[k] = (T)pg;
+      ph *= h;
+      pg *= g;
+    }
+  }
+  __syncthreads();
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  const uint64_t m55 = 0x5555555555555555ull;
+  for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < units; u += step) {
+    P x = ldp<P>(a + u * W);
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+      const uint64_t i = u * W + e;
+      const uint64_t lo = i & m55, hi = (i >> 1) & m55;
+      const int nz = __popcll(lo & hi);
+      const int nxy = __popcll(lo ^ hi);
+      x.v[e] = x.v[e] * (s_h[nxy] * s_g[nz]);
+    }
+    stp(a + u * W, x);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K5: fused k-qubit block on the matrix cores.
+//
+// The block's 4^K components (rows) times W contiguous "column" indices form
+// a tile; C = S * B with S the 4^K x 4^K block PTM.  The tile rows are the
+// digits at positions P[0..K-1]; the columns are the lowest L positions not
+// in P (4^L = W); the remaining positions enumerate tiles.  B is staged in
+// LDS; for K = 3 S^T is staged in LDS too, for K = 4 (256x256) it is read
+// through the L2.  v_mfma_*_16x16x4 operand layout (per lane l of a wave):
+//   A[16x4]: A[l%16][l/16]   B[4x16]: B[l/16][l%16]
+//   D[16x16] in 4 registers: D[MfmaT<T>::row(l/16, i)][l%16]
+// ---------------------------------------------------------------------------
+
+typedef double dx4 __attribute__((ext_vector_type(4)));
+typedef float fx4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct MfmaT;
+// Accumulator row of register v in lane group lk = lane/16 (column lane%16),
+// as measured on gfx950: f32 holds rows 4*lk+v, f64 holds rows lk+4*v.
+template <>
+struct MfmaT<double> {
+  typedef dx4 acc;
+  static __device__ __forceinline__ acc mma(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
+};
+template <>
+struct MfmaT<float> {
+  typedef fx4 acc;
+  static __device__ __forceinline__ acc mma(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
+};
+
+struct BlockGeom {
+  int n;
+  int pos[4];       // block positions, ascending (digit j of the row index)
+  int lpos[4];      // column positions, ascending
+  int rpos[kMaxQubits];  // tile-enumerating positions, ascending
+  int nr;
+  uint64_t tiles;
+};
+
+template <typename T, int K, int L>
+__global__ __launch_bounds__(kBlock) void k_block_mfma(T *__restrict__ a, const T *__restrict__ ST,
+                                                       BlockGeom geo) {
+  constexpr int ROWS = 1 << (2 * K);
+  constexpr int W = 1 << (2 * L);
+  constexpr int PAD = (sizeof(T) == 4 && (W % 32) == 0) ? 16 : 0;
+  constexpr int LDW = W + PAD;
+  constexpr bool ST_IN_LDS = (K <= 3);
+  constexpr int LDR = ROWS + ((sizeof(T) == 4 && (ROWS % 32) == 0) ? 16 : 0);
+  using MT = MfmaT<T>;
+  using Acc = typename MT::acc;
+
+  __shared__ T s_b[ROWS * LDW];
+  __shared__ T s_st[ST_IN_LDS ? ROWS * LDR : 1];
+  __shared__ uint64_t s_offr[ROWS];
+  __shared__ uint64_t s_offc[W];
+
+  const int tid = threadIdx.x;
+  for (int r = tid; r < ROWS; r += kBlock) {
+    uint64_t o = 0;
+    for (int j = 0; j < K; ++j) o += (uint64_t)((r >> (2 * j)) & 3) << (2 * geo.pos[j]);
+    s_offr[r] = o;
+  }
+  for (int c = tid; c < W; c += kBlock) {
+    uint64_t o = 0;
+    for (int j = 0; j < L; ++j) o += (uint64_t)((c >> (2 * j)) & 3) << (2 * geo.lpos[j]);
+    s_offc[c] = o;
+  }
+  if (ST_IN_LDS) {
+    for (int i = tid; i < ROWS * ROWS; i += kBlock) s_st[(i / ROWS) * LDR + (i % ROWS)] = ST[i];
+  }
+  __syncthreads();
+
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const int lr = lane & 15;   // row (A) / column (B, D) within a 16-block
+  const int lk = lane >> 4;   // k index (A, B) / row quad (D)
+
+  // Work split per wave: K=3 -> 16 row-blocks? no: 4 row-blocks x (W/16) col-blocks.
+  constexpr int RB = ROWS / 16;          // row blocks
+  constexpr int CB = W / 16;             // column blocks
+  constexpr int NB = RB * CB;            // 16x16 output blocks per tile
+  constexpr int PER_WAVE = NB / 4;       // blocks per wave (4 waves)
+  static_assert(NB % 4 == 0, "tile must split over 4 waves");
+
+  for (uint64_t t = blockIdx.x; t < geo.tiles; t += gridDim.x) {
+    // tile base: deposit t's digits on the enumerating positions
+    uint64_t base = 0;
+    {
+      uint64_t r = t;
+      for (int j = 0; j < geo.nr; ++j) {
+        base += (r & 3ull) << (2 * geo.rpos[j]);
+        r >>= 2;
+      }
+    }
+    // stage B
+    for (int e = tid; e < ROWS * W; e += kBlock) {
+      const int r = e / W, c = e % W;
+      s_b[r * LDW + c] = a[base + s_offr[r] + s_offc[c]];
+    }
+    __syncthreads();
+
+    Acc acc[PER_WAVE];
+#pragma unroll
+    for (int i = 0; i < PER_WAVE; ++i) acc[i] = Acc{0, 0, 0, 0};
+
+    // output block index ob = wave * PER_WAVE + i -> (rb, cb) row-major over CB
+#pragma unroll 4
+    for (int k0 = 0; k0 < ROWS; k0 += 4) {
+      const int kk = k0 + lk;
+#pragma unroll
+      for (int i = 0; i < PER_WAVE; ++i) {
+        const int ob = wave * PER_WAVE + i;
+        const int rb = ob / CB, cb = ob % CB;
+        const T av = ST_IN_LDS ? s_st[kk * LDR + rb * 16 + lr] : ST[(uint64_t)kk * ROWS + rb * 16 + lr];
+        const T bv = s_b[kk * LDW + cb * 16 + lr];
+        acc[i] = MT::mma(av, bv, acc[i]);
+      }
+    }
+    // write D straight to HBM
+#pragma unroll
+    for (int i = 0; i < PER_WAVE; ++i) {
+      const int ob = wave * PER_WAVE + i;
+      const int rb = ob / CB, cb = ob % CB;
+      const uint64_t oc = s_offc[cb * 16 + lr];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int row = rb * 16 + MT::row(lk, j);
+        a[base + s_offr[row] + oc] = acc[i][j];
+      }
+    }
+    __syncthreads();  // s_b is restaged by the next tile
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K6: reductions (FP64 accumulation).
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double s_part[kBlock / 64];
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) s_part[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kBlock / 64; ++w) t += s_part[w];
+  return t;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sumsq(const T *__restrict__ a, uint64_t N, double *part) {
+  double s = 0.0;
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += step) {
+    const double v = (double)a[i];
+    s += v * v;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dot(const T *__restrict__ a, const double *__restrict__ b,
+                                                uint64_t N, double *part) {
+  double s = 0.0;
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += step)
+    s += (double)a[i] * b[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_parts(const double *part, int nparts, double *out) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += kBlock) s += part[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) out[0] = s;
+}
+
+// Gather the coefficients with measured qubits' components in the basis
+// support and I on every other qubit. j's digit t (base 2 or 4, most
+// significant first) is measured qubit t.
+struct GatherSpec {
+  int m;
+  int base_log;           // 1 (components {0,k}) or 2 (all four)
+  int comp;               // k for base 2
+  uint64_t stride[kMaxQubits];
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gather(const T *__restrict__ a, uint64_t count, GatherSpec gs,
+                                                   double *out) {
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < count; j += step) {
+    uint64_t idx = 0;
+    for (int t = 0; t < gs.m; ++t) {
+      const int sh = gs.base_log * (gs.m - 1 - t);
+      const uint64_t d = (j >> sh) & ((1ull << gs.base_log) - 1ull);
+      const uint64_t c = gs.base_log == 1 ? (d ? (uint64_t)gs.comp : 0ull) : d;
+      idx += c * gs.stride[t];
+    }
+    out[j] = (double)a[idx];
+  }
+}
+
+// One contraction step over digit t: (pre, din, post) -> (pre, 2, post),
+// out[p,b,q] = sum_x w[b][x] in[p,x,q].
+__global__ __launch_bounds__(kBlock) void k_contract(const double *__restrict__ in, double *__restrict__ out,
+                                                     uint64_t pre, int din, uint64_t post, double w00,
+                                                     double w01, double w02, double w03, double w10,
+                                                     double w11, double w12, double w13) {
+  const uint64_t total = pre * 2 * post;
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t o = (uint64_t)blockIdx.x * kBlock + threadIdx.x; o < total; o += step) {
+    const uint64_t q = o % post;
+    const uint64_t b = (o / post) & 1ull;
+    const uint64_t p = o / (2 * post);
+    const double *src = in + p * din * post + q;
+    const double w0 = b ? w10 : w00, w1 = b ? w11 : w01, w2 = b ? w12 : w02, w3 = b ? w13 : w03;
+    double s = w0 * src[0] + w1 * src[post];
+    if (din == 4) s += w2 * src[2 * post] + w3 * src[3 * post];
+    out[o] = s;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_chop(T *__restrict__ a, uint64_t N, T thr) {
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += step)
+    if (fabs((double)a[i]) < (double)thr) a[i] = T(0);
+}
+
+template <typename TS, typename TD>
+__global__ __launch_bounds__(kBlock) void k_convert(const TS *__restrict__ s, TD *__restrict__ d, uint64_t N) {
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += step) d[i] = (TD)s[i];
+}
+
+// ---------------------------------------------------------------------------
+// K7: Pauli -> matrix conversion.  Per qubit, (I,X,Y,Z) -> (I+Z, X-iY, X+iY,
+// I-Z) on complex values; then mu_q = 2 r_q + c_q is scattered to
+// (row, col) with qubit 0 the most significant bit.
+// ---------------------------------------------------------------------------
+
+template <typename TS>
+__global__ __launch_bounds__(kBlock) void k_dense_pass(const TS *__restrict__ in, double2 *__restrict__ out,
+                                                       uint64_t groups, int p, int in_complex) {
+  const uint64_t S = pow4(p);
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < groups; g += step) {
+    const uint64_t hi = g >> (2 * p), lo = g & (S - 1);
+    const uint64_t b = hi * 4 * S + lo;
+    double2 x[4];
+    for (int c = 0; c < 4; ++c) {
+      if (in_complex) {
+        x[c] = reinterpret_cast<const double2 *>(in)[b + c * S];
+      } else {
+        x[c] = make_double2((double)in[b + c * S], 0.0);
+      }
+    }
+    // mu0 = I + Z; mu1 = X - iY; mu2 = X + iY; mu3 = I - Z
+    out[b] = make_double2(x[0].x + x[3].x, x[0].y + x[3].y);
+    out[b + S] = make_double2(x[1].x + x[2].y, x[1].y - x[2].x);
+    out[b + 2 * S] = make_double2(x[1].x - x[2].y, x[1].y + x[2].x);
+    out[b + 3 * S] = make_double2(x[0].x - x[3].x, x[0].y - x[3].y);
+  }
+}
+
+__device__ __forceinline__ uint64_t spread_bits(uint64_t x) {
+  // bit i -> bit 2i (x < 2^32)
+  x &= 0xffffffffull;
+  x = (x | (x << 16)) & 0x0000ffff0000ffffull;
+  x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
+  x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
+  x = (x | (x << 2)) & 0x3333333333333333ull;
+  x = (x | (x << 1)) & 0x5555555555555555ull;
+  return x;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dense_scatter(const double2 *__restrict__ b, double2 *__restrict__ out,
+                                                          int n) {
+  const uint64_t dim = 1ull << n;
+  const uint64_t total = dim * dim;
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t o = (uint64_t)blockIdx.x * kBlock + threadIdx.x; o < total; o += step) {
+    const uint64_t row = o >> n, col = o & (dim - 1);
+    out[o] = b[(spread_bits(row) << 1) | spread_bits(col)];
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+
+struct ProfSlot {
+  uint64_t launches = 0;
+  double ms = 0.0;
+};
+
+struct ProfPending {
+  hipEvent_t a, b;
+  int fam;
+};
+
+struct dm_state {
+  int n = 0;
+  int dtype = DM_FP64;
+  uint64_t N = 0;
+  int device = 0;
+  int cu_count = 256;
+  void *d_a = nullptr;
+  hipStream_t stream = nullptr;
+  // scratch
+  double *d_red = nullptr;      // reduction partials + results
+  size_t red_cap = 0;           // doubles
+  void *d_mat = nullptr;        // block matrices (transposed, storage dtype)
+  size_t mat_cap = 0;           // bytes
+  double *d_aux = nullptr;      // gathered vectors / overlap operand
+  size_t aux_cap = 0;           // bytes
+  // profiling
+  bool prof = false;
+  int prof_depth = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  ProfSlot slots[kFamilies];
+  std::vector<ProfPending> pending;      // recorded, not yet resolved
+  std::vector<hipEvent_t> spare_events;  // reusable
+  hipEvent_t cur_a = nullptr;
+};
+
+namespace {
+
+int grid_for(const dm_state *h, uint64_t units) {
+  const uint64_t want = (units + kBlock - 1) / kBlock;
+  const uint64_t cap = (uint64_t)h->cu_count * 64;  // >= 16 waves per SIMD worth of blocks
+  return (int)std::max<uint64_t>(1, std::min(want, cap));
+}
+
+int ensure(void **p, size_t *cap, size_t bytes) {
+  if (*cap >= bytes) return DM_OK;
+  if (*p) hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  HIP_TRY(hipMalloc(p, bytes));
+  *cap = bytes;
+  return DM_OK;
+}
+
+int check_handle(const dm_state *h) {
+  if (!h) return fail(DM_ERR_ARG, "null handle");
+  if (!h->d_a) return fail(DM_ERR_STATE, "handle has no device state");
+  return DM_OK;
+}
+
+int check_qubit(const dm_state *h, int q) {
+  if (q < 0 || q >= h->n) return fail(DM_ERR_ARG, "qubit %d out of range for %d qubits", q, h->n);
+  return DM_OK;
+}
+
+hipEvent_t take_event(dm_state *h) {
+  if (!h->spare_events.empty()) {
+    hipEvent_t e = h->spare_events.back();
+    h->spare_events.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets the outermost library operation with a pair of HIP events on the
+// handle's stream (nested scopes of composite operations fold into the outer
+// one).  Nothing synchronises here; dm_profile_read resolves the pairs.
+struct ProfScope {
+  dm_state *h;
+  int fam;
+  bool outer;
+  hipEvent_t a = nullptr;
+  ProfScope(dm_state *h_, int fam_) : h(h_), fam(fam_), outer(h_->prof_depth++ == 0) {
+    if (h->prof && outer) {
+      a = take_event(h);
+      if (a) hipEventRecord(a, h->stream);
+    }
+  }
+  ~ProfScope() {
+    h->prof_depth--;
+    if (!h->prof || !outer || !a) return;
+    hipEvent_t b = take_event(h);
+    if (!b) return;
+    hipEventRecord(b, h->stream);
+    h->pending.push_back({a, b, fam});
+  }
+};
+
+int resolve_profile(dm_state *h) {
+  if (h->pending.empty()) return DM_OK;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  for (auto &pp : h->pending) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, pp.a, pp.b);
+    h->slots[pp.fam].launches += 1;
+    h->slots[pp.fam].ms += ms;
+    h->spare_events.push_back(pp.a);
+    h->spare_events.push_back(pp.b);
+  }
+  h->pending.clear();
+  return DM_OK;
+}
+
+int last_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(DM_ERR_HIP, "%s launch failed: %s", what, hipGetErrorString(e));
+  return DM_OK;
+}
+
+template <typename T>
+int launch_1q(dm_state *h, int q, const double *R) {
+  const int p = h->n - 1 - q;
+  bool use_i = R[0] != 0.0 || R[4] != 0.0 || R[8] != 0.0;
+  Rows3<T> m = make_rows3<T>(R);
+  T *a = static_cast<T *>(h->d_a);
+  constexpr int W = 16 / sizeof(T);
+  ProfScope ps(h, 0);
+  if (p == 0) {
+    const uint64_t groups = h->N / 4;
+    if (use_i)
+      hipLaunchKernelGGL((k_1q_p0<T, true>), dim3(grid_for(h, groups)), dim3(kBlock), 0, h->stream, a, groups, m);
+    else
+      hipLaunchKernelGGL((k_1q_p0<T, false>), dim3(grid_for(h, groups)), dim3(kBlock), 0, h->stream, a, groups, m);
+  } else {
+    const uint64_t S = pow4(p);
+    const uint64_t units = h->N / 4 / W;
+    int sh = 2 * p - (W == 2 ? 1 : 2);
+    if (use_i)
+      hipLaunchKernelGGL((k_1q_wide<T, true>), dim3(grid_for(h, units)), dim3(kBlock), 0, h->stream, a, units, sh, S, m);
+    else
+      hipLaunchKernelGGL((k_1q_wide<T, false>), dim3(grid_for(h, units)), dim3(kBlock), 0, h->stream, a, units, sh, S, m);
+  }
+  return last_launch("k_1q");
+}
+
+template <typename T, bool CTRL_HI, bool SCALED>
+int launch_cx_t(dm_state *h, int ph, int pl, const Scale16<T> &sc) {
+  T *a = static_cast<T *>(h->d_a);
+  constexpr int W = 16 / sizeof(T);
+  const uint64_t Sh = pow4(ph), Sl = pow4(pl);
+  const int sh_mid = 2 * (ph - pl - 1);
+  if (pl == 0) {
+    const uint64_t groups = h->N / 16;
+    hipLaunchKernelGGL((k_cx_p0<T, CTRL_HI, SCALED>), dim3(grid_for(h, groups)), dim3(kBlock), 0, h->stream, a,
+                       groups, sh_mid, Sh, sc);
+  } else {
+    const uint64_t units = h->N / 16 / W;
+    const int sh_lo = 2 * pl - (W == 2 ? 1 : 2);
+    hipLaunchKernelGGL((k_cx_wide<T, CTRL_HI, SCALED>), dim3(grid_for(h, units)), dim3(kBlock), 0, h->stream, a,
+                       units, sh_lo, sh_mid, Sh, Sl, sc);
+  }
+  return last_launch("k_cx");
+}
+
+template <typename T>
+int launch_cx(dm_state *h, int c, int t, const double *scale16) {
+  const int pc = h->n - 1 - c, pt = h->n - 1 - t;
+  const bool ctrl_hi = pc > pt;
+  const int ph = std::max(pc, pt), pl = std::min(pc, pt);
+  Scale16<T> sc;
+  bool scaled = false;
+  for (int i = 0; i < 16; ++i) {
+    sc.s[i] = scale16 ? (T)scale16[i] : T(1);
+    if (scale16 && scale16[i] != 1.0 && i != 0) scaled = true;
+  }
+  ProfScope ps(h, 1);
+  if (ctrl_hi) {
+    return scaled ? launch_cx_t<T, true, true>(h, ph, pl, sc) : launch_cx_t<T, true, false>(h, ph, pl, sc);
+  }
+  return scaled ? launch_cx_t<T, false, true>(h, ph, pl, sc) : launch_cx_t<T, false, false>(h, ph, pl, sc);
+}
+
+template <typename T>
+int launch_2q(dm_state *h, int qa, int qb, const double *R) {
+  const int pa = h->n - 1 - qa, pb = h->n - 1 - qb;
+  const int ph = std::max(pa, pb), pl = std::min(pa, pb);
+  // pair index in position order (i_hi, i_lo); R uses (i_qa, i_qb)
+  Mat16<T> M;
+  uint32_t rowmask = 0;
+  for (int r = 0; r < 16; ++r) {
+    const int rh = r >> 2, rl = r & 3;
+    const int ra = pa > pb ? rh : rl, rb = pa > pb ? rl : rh;
+    const int rsrc = 4 * ra + rb;
+    bool ident = true;
+    for (int c = 0; c < 16; ++c) {
+      const int ch = c >> 2, cl = c & 3;
+      const int ca = pa > pb ? ch : cl, cb = pa > pb ? cl : ch;
+      const double v = R[16 * rsrc + 4 * ca + cb];
+      M.m[r][c] = (T)v;
+      if (v != (r == c ? 1.0 : 0.0)) ident = false;
+    }
+    if (!ident) rowmask |= 1u << r;
+  }
+  const uint64_t groups = h->N / 16;
+  const int sh_lo = 2 * pl, sh_mid = 2 * (ph - pl - 1);
+  ProfScope ps(h, 2);
+  hipLaunchKernelGGL((k_2q_dense<T>), dim3(grid_for(h, groups)), dim3(kBlock), 0, h->stream,
+                     static_cast<T *>(h->d_a), groups, sh_lo, sh_mid, pow4(ph), pow4(pl), M, rowmask);
+  return last_launch("k_2q_dense");
+}
+
+template <typename T, int M>
+int launch_tile_maps(dm_state *h, const int *pos, const double (*A)[4][4]) {
+  TileMaps<T, M> tm;
+  for (int j = 0; j < M; ++j) {
+    tm.pos[j] = pos[j];
+    for (int r = 0; r < 4; ++r)
+      for (int c = 0; c < 4; ++c) tm.A[j][r][c] = (T)A[j][r][c];
+  }
+  const uint64_t units = h->N >> (2 * M);
+  hipLaunchKernelGGL((k_tile_maps<T, M>), dim3(grid_for(h, units)), dim3(kBlock), 0, h->stream,
+                     static_cast<T *>(h->d_a), units, tm);
+  return last_launch("k_tile_maps");
+}
+
+template <typename T>
+int launch_memory_noise(dm_state *h, double f, double p, double g) {
+  const double od = sqrt(g) * f;
+  const double dd = (1 - g) * (2 * p - 1);
+  ProfScope ps(h, 3);
+  if (dd == 0.0) {
+    constexpr int W = 16 / sizeof(T);
+    const uint64_t units = h->N / W;
+    hipLaunchKernelGGL((k_diag_pow<T>), dim3(grid_for(h, units)), dim3(kBlock), 0, h->stream,
+                       static_cast<T *>(h->d_a), units, h->n, od, g);
+    return last_launch("k_diag_pow");
+  }
+  double A[3][4][4];
+  for (int j = 0; j < 3; ++j) {
+    memset(A[j], 0, sizeof(A[j]));
+    A[j][0][0] = 1.0;
+    A[j][1][1] = od;
+    A[j][2][2] = od;
+    A[j][3][3] = g;
+    A[j][3][0] = dd;
+  }
+  // position 0 (the last qubit): contiguous quads, the vectorised 1q kernel
+  {
+    const double R[12] = {0, od, 0, 0, 0, 0, od, 0, dd, 0, 0, g};
+    int rc = launch_1q<T>(h, h->n - 1, R);
+    if (rc) return rc;
+  }
+  int p0 = 1;
+  while (p0 < h->n) {
+    const int m = std::min(3, h->n - p0);
+    int pos[3] = {p0, p0 + 1, p0 + 2};
+    int rc = m == 3 ? launch_tile_maps<T, 3>(h, pos, A)
+                    : m == 2 ? launch_tile_maps<T, 2>(h, pos, A) : launch_tile_maps<T, 1>(h, pos, A);
+    if (rc) return rc;
+    p0 += m;
+  }
+  return DM_OK;
+}
+
+template <typename T>
+int launch_diag_tab(dm_state *h, const double *tab_by_pos, bool write_only, double scale) {
+  Tab4 tab;
+  memset(&tab, 0, sizeof(tab));
+  memcpy(tab.t, tab_by_pos, sizeof(double) * 4 * h->n);
+  constexpr int W = 16 / sizeof(T);
+  const uint64_t units = h->N / W;
+  ProfScope ps(h, write_only ? 7 : 4);
+  if (write_only)
+    hipLaunchKernelGGL((k_diag_tab<T, true>), dim3(grid_for(h, units)), dim3(kBlock), 0, h->stream,
+                       static_cast<T *>(h->d_a), units, h->n, tab, scale);
+  else
+    hipLaunchKernelGGL((k_diag_tab<T, false>), dim3(grid_for(h, units)), dim3(kBlock), 0, h->stream,
+                       static_cast<T *>(h->d_a), units, h->n, tab, scale);
+  return last_launch("k_diag_tab");
+}
+
+template <typename T, int K, int L>
+int launch_block_t(dm_state *h, const int *pos_sorted, const T *dST) {
+  BlockGeom geo;
+  memset(&geo, 0, sizeof(geo));
+  geo.n = h->n;
+  bool inblk[kMaxQubits] = {false};
+  for (int j = 0; j < K; ++j) {
+    geo.pos[j] = pos_sorted[j];
+    inblk[pos_sorted[j]] = true;
+  }
+  int nl = 0;
+  for (int p = 0; p < h->n && nl < L; ++p)
+    if (!inblk[p]) {
+      geo.lpos[nl++] = p;
+      inblk[p] = true;
+    }
+  if (nl < L) return fail(DM_ERR_ARG, "block needs at least %d qubits", K + L);
+  geo.nr = 0;
+  for (int p = 0; p < h->n; ++p)
+    if (!inblk[p]) geo.rpos[geo.nr++] = p;
+  geo.tiles = 1ull << (2 * geo.nr);
+  const int grid = (int)std::min<uint64_t>(geo.tiles, (uint64_t)h->cu_count * 8);
+  hipLaunchKernelGGL((k_block_mfma<T, K, L>), dim3(grid), dim3(kBlock), 0, h->stream, static_cast<T *>(h->d_a),
+                     dST, geo);
+  return last_launch("k_block_mfma");
+}
+
+// Block matrix in kernel order: S is indexed by sum_j i_{qubits[j]} 4^j; the
+// kernel's row digit j is the j-th ascending position.  Writes ST[col][row]
+// (rows*rows values of T) and the ascending positions.
+template <typename T>
+void prep_block_matrix(int n, const int *qubits, int k, const double *S, T *st, int *pos_sorted) {
+  const int rows = 1 << (2 * k);
+  int pos[4], order[4];
+  for (int j = 0; j < k; ++j) {
+    pos[j] = n - 1 - qubits[j];
+    order[j] = j;
+  }
+  std::sort(order, order + k, [&](int x, int y) { return pos[x] < pos[y]; });
+  for (int j = 0; j < k; ++j) pos_sorted[j] = pos[order[j]];
+  std::vector<int> remap(rows);
+  for (int r = 0; r < rows; ++r) {
+    int src = 0;
+    for (int j = 0; j < k; ++j) src |= ((r >> (2 * j)) & 3) << (2 * order[j]);
+    remap[r] = src;
+  }
+  for (int r = 0; r < rows; ++r)
+    for (int c = 0; c < rows; ++c) st[(size_t)c * rows + r] = (T)S[(size_t)remap[r] * rows + remap[c]];
+}
+
+template <typename T>
+int launch_block_dev(dm_state *h, int k, const int *pos_sorted, const T *dST) {
+  ProfScope ps(h, k == 4 ? 8 : 5);
+  if (k == 3) return launch_block_t<T, 3, 3>(h, pos_sorted, dST);
+  if (k == 4) return launch_block_t<T, 4, 2>(h, pos_sorted, dST);
+  return fail(DM_ERR_ARG, "block size k=%d not supported (3 or 4)", k);
+}
+
+template <typename T>
+int launch_block(dm_state *h, const int *qubits, int k, const double *S) {
+  const int rows = 1 << (2 * k);
+  std::vector<T> st((size_t)rows * rows);
+  int pos_sorted[4];
+  prep_block_matrix<T>(h->n, qubits, k, S, st.data(), pos_sorted);
+  int rc = ensure(&h->d_mat, &h->mat_cap, st.size() * sizeof(T));
+  if (rc) return rc;
+  // synchronous copy: the staging vector is released on return, and the
+  // previous block kernel reading d_mat has to finish first
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipMemcpy(h->d_mat, st.data(), st.size() * sizeof(T), hipMemcpyHostToDevice));
+  return launch_block_dev<T>(h, k, pos_sorted, static_cast<const T *>(h->d_mat));
+}
+
+int reserve_red(dm_state *h, size_t doubles) {
+  return ensure(reinterpret_cast<void **>(&h->d_red), &h->red_cap, doubles * sizeof(double));
+}
+
+template <typename T>
+int do_sumsq(dm_state *h, double *out) {
+  const int grid = std::min(grid_for(h, h->N), 4096);
+  int rc = reserve_red(h, (size_t)grid + 8);
+  if (rc) return rc;
+  ProfScope ps(h, 6);
+  hipLaunchKernelGGL((k_sumsq<T>), dim3(grid), dim3(kBlock), 0, h->stream, static_cast<const T *>(h->d_a), h->N,
+                     h->d_red);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, h->stream, h->d_red, grid, h->d_red + grid);
+  rc = last_launch("k_sumsq");
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(out, h->d_red + grid, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return DM_OK;
+}
+
+template <typename T>
+int do_dot(dm_state *h, const double *d_b, double *out) {
+  const int grid = std::min(grid_for(h, h->N), 4096);
+  int rc = reserve_red(h, (size_t)grid + 8);
+  if (rc) return rc;
+  ProfScope ps(h, 6);
+  hipLaunchKernelGGL((k_dot<T>), dim3(grid), dim3(kBlock), 0, h->stream, static_cast<const T *>(h->d_a), d_b,
+                     h->N, h->d_red);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, h->stream, h->d_red, grid, h->d_red + grid);
+  rc = last_launch("k_dot");
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(out, h->d_red + grid, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return DM_OK;
+}
+
+template <typename T>
+int do_probabilities(dm_state *h, const int *qubits, int m, char basis, const double *nvec, double err,
+                     double *out) {
+  GatherSpec gs;
+  memset(&gs, 0, sizeof(gs));
+  gs.m = m;
+  double w[2][4] = {{1, 0, 0, 0}, {1, 0, 0, 0}};
+  if (basis == 'N') {
+    gs.base_log = 2;
+    for (int k = 0; k < 3; ++k) {
+      w[0][k + 1] = nvec[k] * err;
+      w[1][k + 1] = -nvec[k] * err;
+    }
+  } else {
+    gs.base_log = 1;
+    gs.comp = basis == 'X' ? 1 : basis == 'Y' ? 2 : 3;
+    w[0][1] = err;
+    w[1][1] = -err;
+  }
+  for (int t = 0; t < m; ++t) gs.stride[t] = pow4(h->n - 1 - qubits[t]);
+  const uint64_t count = 1ull << (gs.base_log * m);
+  const uint64_t outn = 1ull << m;
+  // aux holds two ping-pong buffers of `count` doubles
+  int rc = ensure(reinterpret_cast<void **>(&h->d_aux), &h->aux_cap, 2 * count * sizeof(double));
+  if (rc) return rc;
+  double *buf0 = h->d_aux, *buf1 = h->d_aux + count;
+  ProfScope ps(h, 6);
+  hipLaunchKernelGGL((k_gather<T>), dim3(grid_for(h, count)), dim3(kBlock), 0, h->stream,
+                     static_cast<const T *>(h->d_a), count, gs, buf0);
+  double *src = buf0, *dst = buf1;
+  const uint64_t din = 1ull << gs.base_log;
+  for (int t = 0; t < m; ++t) {
+    const uint64_t pre = 1ull << t;
+    uint64_t post = 1;
+    for (int r = t + 1; r < m; ++r) post *= din;
+    hipLaunchKernelGGL(k_contract, dim3(grid_for(h, pre * 2 * post)), dim3(kBlock), 0, h->stream, src, dst, pre,
+                       (int)din, post, w[0][0], w[0][1], w[0][2], w[0][3], w[1][0], w[1][1], w[1][2], w[1][3]);
+    std::swap(src, dst);
+  }
+  rc = last_launch("k_gather/k_contract");
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(out, src, outn * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  // marginal over the unmeasured qubits: each contributes a factor 2 (sum of
+  // its two outcome rows on the identity component)
+  const double fac = ldexp(1.0, h->n - m);
+  for (uint64_t i = 0; i < outn; ++i) out[i] *= fac;
+  return DM_OK;
+}
+
+template <typename T>
+int do_to_dense(dm_state *h, double *out_c128) {
+  const uint64_t N = h->N;
+  const size_t bytes = N * sizeof(double2);
+  double2 *b0 = nullptr, *b1 = nullptr;
+  HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&b0), bytes, h->stream));
+  hipError_t e1 = hipMallocAsync(reinterpret_cast<void **>(&b1), bytes, h->stream);
+  if (e1 != hipSuccess) {
+    hipFreeAsync(b0, h->stream);
+    return fail(DM_ERR_HIP, "hipMallocAsync(dense scratch) failed: %s", hipGetErrorString(e1));
+  }
+  const uint64_t groups = N / 4;
+  ProfScope ps(h, 6);
+  // qubit order 0..n-1, i.e. position n-1 down to 0 (any order gives the same result)
+  const double2 *src = nullptr;
+  double2 *dst = b0;
+  for (int q = 0; q < h->n; ++q) {
+    const int p = h->n - 1 - q;
+    if (q == 0)
+      hipLaunchKernelGGL((k_dense_pass<T>), dim3(grid_for(h, groups)), dim3(kBlock), 0, h->stream,
+                         static_cast<const T *>(h->d_a), dst, groups, p, 0);
+    else
+      hipLaunchKernelGGL((k_dense_pass<double>), dim3(grid_for(h, groups)), dim3(kBlock), 0, h->stream,
+                         reinterpret_cast<const double *>(src), dst, groups, p, 1);
+    src = dst;
+    dst = (dst == b0) ? b1 : b0;
+  }
+  hipLaunchKernelGGL(k_dense_scatter, dim3(grid_for(h, N)), dim3(kBlock), 0, h->stream, src, dst, h->n);
+  int rc = last_launch("k_dense");
+  if (rc == DM_OK) {
+    hipError_t e = hipMemcpyAsync(out_c128, dst, bytes, hipMemcpyDeviceToHost, h->stream);
+    if (e != hipSuccess) rc = fail(DM_ERR_HIP, "dense download failed: %s", hipGetErrorString(e));
+  }
+  hipFreeAsync(b0, h->stream);
+  hipFreeAsync(b1, h->stream);
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return rc;
+}
+
+int param_count(const dm_state *h, const dm_op &op) {
+  switch (op.kind) {
+    case DM_OP_1Q: return 12;
+    case DM_OP_CX: return 0;
+    case DM_OP_CX_SCALED: return 16;
+    case DM_OP_2Q: return 256;
+    case DM_OP_BLOCK: return 1 << (4 * op.k);
+    case DM_OP_MEMORY_NOISE: return 3;
+    case DM_OP_DIAG: return 4 * h->n;
+    default: return -1;
+  }
+}
+
+int validate_op(const dm_state *h, const dm_op &op, int i, int64_t nparams) {
+  const int need = param_count(h, op);
+  if (need < 0) return fail(DM_ERR_ARG, "op %d: unknown kind %d", i, op.kind);
+  int nq = 0;
+  switch (op.kind) {
+    case DM_OP_1Q: nq = 1; break;
+    case DM_OP_CX: case DM_OP_CX_SCALED: case DM_OP_2Q: nq = 2; break;
+    case DM_OP_BLOCK:
+      if (op.k != 3 && op.k != 4) return fail(DM_ERR_ARG, "op %d: block size %d not supported", i, op.k);
+      if (h->n < op.k + (op.k == 3 ? 3 : 2)) return fail(DM_ERR_ARG, "op %d: block too large for %d qubits", i, h->n);
+      nq = op.k;
+      break;
+    default: nq = 0;
+  }
+  for (int j = 0; j < nq; ++j) {
+    if (op.qubits[j] < 0 || op.qubits[j] >= h->n)
+      return fail(DM_ERR_ARG, "op %d: qubit %d out of range for %d qubits", i, op.qubits[j], h->n);
+    for (int m = 0; m < j; ++m)
+      if (op.qubits[m] == op.qubits[j]) return fail(DM_ERR_ARG, "op %d: repeated qubit %d", i, op.qubits[j]);
+  }
+  if (need > 0 && (op.param < 0 || op.param + need > nparams))
+    return fail(DM_ERR_ARG, "op %d: parameters [%lld, %lld) outside the %lld given", i, (long long)op.param,
+                (long long)(op.param + need), (long long)nparams);
+  return DM_OK;
+}
+
+template <typename T>
+int run_program(dm_state *h, const dm_op *ops, int nops, const double *params, int64_t nparams) {
+  size_t pool = 0;
+  for (int i = 0; i < nops; ++i) {
+    if (int rc = validate_op(h, ops[i], i, nparams)) return rc;
+    if (ops[i].kind == DM_OP_BLOCK) pool += (size_t)1 << (4 * ops[i].k);
+  }
+  std::vector<size_t> offs(nops, 0);
+  std::vector<int> pos(4 * (size_t)nops, 0);
+  if (pool) {
+    std::vector<T> staging(pool);
+    size_t off = 0;
+    for (int i = 0; i < nops; ++i) {
+      if (ops[i].kind != DM_OP_BLOCK) continue;
+      prep_block_matrix<T>(h->n, ops[i].qubits, ops[i].k, params + ops[i].param, staging.data() + off,
+                           &pos[4 * (size_t)i]);
+      offs[i] = off;
+      off += (size_t)1 << (4 * ops[i].k);
+    }
+    if (int rc = ensure(&h->d_mat, &h->mat_cap, pool * sizeof(T))) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));  // earlier block kernels may still read d_mat
+    HIP_TRY(hipMemcpy(h->d_mat, staging.data(), pool * sizeof(T), hipMemcpyHostToDevice));
+  }
+  const T *dmat = static_cast<const T *>(h->d_mat);
+  for (int i = 0; i < nops; ++i) {
+    const dm_op &op = ops[i];
+    const double *pv = params ? params + op.param : nullptr;
+    int rc = DM_OK;
+    switch (op.kind) {
+      case DM_OP_1Q: rc = launch_1q<T>(h, op.qubits[0], pv); break;
+      case DM_OP_CX: rc = launch_cx<T>(h, op.qubits[0], op.qubits[1], nullptr); break;
+      case DM_OP_CX_SCALED: rc = launch_cx<T>(h, op.qubits[0], op.qubits[1], pv); break;
+      case DM_OP_2Q: rc = launch_2q<T>(h, op.qubits[0], op.qubits[1], pv); break;
+      case DM_OP_BLOCK: rc = launch_block_dev<T>(h, op.k, &pos[4 * (size_t)i], dmat + offs[i]); break;
+      case DM_OP_MEMORY_NOISE:
+        if (!(pv[0] == 1.0 && pv[2] == 1.0)) rc = launch_memory_noise<T>(h, pv[0], pv[1], pv[2]);
+        break;
+      case DM_OP_DIAG: {
+        double tab[4 * kMaxQubits];
+        for (int p = 0; p < h->n; ++p)
+          for (int c = 0; c < 4; ++c) tab[4 * p + c] = pv[4 * (h->n - 1 - p) + c];
+        rc = launch_diag_tab<T>(h, tab, false, 1.0);
+        break;
+      }
+      default: rc = fail(DM_ERR_ARG, "op %d: unknown kind", i);
+    }
+    if (rc) return rc;
+  }
+  return DM_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+#define DISPATCH(h, fn, ...) ((h)->dtype == DM_FP64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+extern "C" {
+
+int dm_last_error(char *buf, size_t len) {
+  if (!buf || len == 0) return DM_ERR_ARG;
+  snprintf(buf, len, "%s", g_err);
+  return DM_OK;
+}
+
+int dm_device_info(int device, int *n_devices, int *cu_count, uint64_t *hbm_bytes) {
+  int nd = 0;
+  HIP_TRY(hipGetDeviceCount(&nd));
+  if (n_devices) *n_devices = nd;
+  if (device < 0 || device >= nd) return fail(DM_ERR_ARG, "device %d not present (%d visible)", device, nd);
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  if (cu_count) *cu_count = prop.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = (uint64_t)prop.totalGlobalMem;
+  return DM_OK;
+}
+
+int dm_create(int n_qubits, int dtype, dm_state **out) {
+  if (!out) return fail(DM_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (n_qubits < 1 || n_qubits > kMaxQubits)
+    return fail(DM_ERR_ARG, "number of qubits %d outside [1, %d]", n_qubits, kMaxQubits);
+  if (dtype != DM_FP64 && dtype != DM_FP32) return fail(DM_ERR_ARG, "unknown dtype %d", dtype);
+  dm_state *h = new dm_state();
+  h->n = n_qubits;
+  h->dtype = dtype;
+  h->N = pow4(n_qubits);
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) {
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, h->device);
+    if (e == hipSuccess) h->cu_count = prop.multiProcessorCount;
+  }
+  if (e == hipSuccess) e = hipStreamCreate(&h->stream);
+  const size_t es = dtype == DM_FP64 ? sizeof(double) : sizeof(float);
+  if (e == hipSuccess) e = hipMalloc(&h->d_a, h->N * es);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+  if (e != hipSuccess) {
+    const int rc = fail(DM_ERR_HIP, "dm_create(%d qubits, %s): %s", n_qubits, dtype ? "fp32" : "fp64",
+                        hipGetErrorString(e));
+    dm_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return DM_OK;
+}
+
+int dm_destroy(dm_state *h) {
+  if (!h) return DM_OK;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->d_a) hipFree(h->d_a);
+  if (h->d_red) hipFree(h->d_red);
+  if (h->d_mat) hipFree(h->d_mat);
+  if (h->d_aux) hipFree(h->d_aux);
+  if (h->ev0) hipEventDestroy(h->ev0);
+  if (h->ev1) hipEventDestroy(h->ev1);
+  for (auto &pp : h->pending) {
+    hipEventDestroy(pp.a);
+    hipEventDestroy(pp.b);
+  }
+  for (auto e : h->spare_events) hipEventDestroy(e);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return DM_OK;
+}
+
+int dm_num_qubits(const dm_state *h, int *n) {
+  if (!h || !n) return fail(DM_ERR_ARG, "null argument");
+  *n = h->n;
+  return DM_OK;
+}
+
+int dm_dtype(const dm_state *h, int *dtype) {
+  if (!h || !dtype) return fail(DM_ERR_ARG, "null argument");
+  *dtype = h->dtype;
+  return DM_OK;
+}
+
+int dm_device_ptr(const dm_state *h, void **ptr) {
+  if (int rc = check_handle(h)) return rc;
+  if (!ptr) return fail(DM_ERR_ARG, "null argument");
+  *ptr = h->d_a;
+  return DM_OK;
+}
+
+int dm_init_product(dm_state *h, const double *per_qubit4, double scale) {
+  if (int rc = check_handle(h)) return rc;
+  if (!per_qubit4) return fail(DM_ERR_ARG, "null per-qubit vectors");
+  // table row p = position p = qubit n-1-p
+  double tab[4 * kMaxQubits];
+  for (int p = 0; p < h->n; ++p)
+    for (int c = 0; c < 4; ++c) tab[4 * p + c] = per_qubit4[4 * (h->n - 1 - p) + c];
+  return DISPATCH(h, launch_diag_tab, h, tab, true, scale);
+}
+
+int dm_upload(dm_state *h, const double *host) {
+  if (int rc = check_handle(h)) return rc;
+  if (!host) return fail(DM_ERR_ARG, "null host buffer");
+  ProfScope ps(h, 7);
+  if (h->dtype == DM_FP64) {
+    HIP_TRY(hipMemcpyAsync(h->d_a, host, h->N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  } else {
+    double *tmp = nullptr;
+    HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&tmp), h->N * sizeof(double), h->stream));
+    HIP_TRY(hipMemcpyAsync(tmp, host, h->N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(h, h->N)), dim3(kBlock), 0, h->stream, tmp,
+                       static_cast<float *>(h->d_a), h->N);
+    HIP_TRY(hipFreeAsync(tmp, h->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return last_launch("upload");
+}
+
+static int download_impl(dm_state *h, double *host) {
+  if (h->dtype == DM_FP64) {
+    HIP_TRY(hipMemcpyAsync(host, h->d_a, h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  } else {
+    double *tmp = nullptr;
+    HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&tmp), h->N * sizeof(double), h->stream));
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(h, h->N)), dim3(kBlock), 0, h->stream,
+                       static_cast<const float *>(h->d_a), tmp, h->N);
+    HIP_TRY(hipMemcpyAsync(host, tmp, h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipFreeAsync(tmp, h->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return last_launch("download");
+}
+
+int dm_download(dm_state *h, double *host) {
+  if (int rc = check_handle(h)) return rc;
+  if (!host) return fail(DM_ERR_ARG, "null host buffer");
+  return download_impl(h, host);
+}
+
+int dm_download_chop(dm_state *h, double thr, double *host) {
+  if (int rc = check_handle(h)) return rc;
+  if (!host) return fail(DM_ERR_ARG, "null host buffer");
+  if (thr > 0) {
+    if (h->dtype == DM_FP64)
+      hipLaunchKernelGGL((k_chop<double>), dim3(grid_for(h, h->N)), dim3(kBlock), 0, h->stream,
+                         static_cast<double *>(h->d_a), h->N, thr);
+    else
+      hipLaunchKernelGGL((k_chop<float>), dim3(grid_for(h, h->N)), dim3(kBlock), 0, h->stream,
+                         static_cast<float *>(h->d_a), h->N, (float)thr);
+  }
+  return download_impl(h, host);
+}
+
+int dm_read_coeffs(dm_state *h, const uint64_t *index, int k, double *out) {
+  if (int rc = check_handle(h)) return rc;
+  if (k < 0 || (k > 0 && (!index || !out))) return fail(DM_ERR_ARG, "bad index list");
+  const size_t es = h->dtype == DM_FP64 ? 8 : 4;
+  for (int i = 0; i < k; ++i) {
+    if (index[i] >= h->N) return fail(DM_ERR_ARG, "coefficient index %llu out of range", (unsigned long long)index[i]);
+    if (h->dtype == DM_FP64) {
+      HIP_TRY(hipMemcpyAsync(out + i, static_cast<char *>(h->d_a) + index[i] * es, es, hipMemcpyDeviceToHost,
+                             h->stream));
+    } else {
+      float f = 0.f;
+      HIP_TRY(hipMemcpyAsync(&f, static_cast<char *>(h->d_a) + index[i] * es, es, hipMemcpyDeviceToHost, h->stream));
+      HIP_TRY(hipStreamSynchronize(h->stream));
+      out[i] = f;
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return DM_OK;
+}
+
+int dm_apply_1q(dm_state *h, int q, const double R[12]) {
+  if (int rc = check_handle(h)) return rc;
+  if (int rc = check_qubit(h, q)) return rc;
+  if (!R) return fail(DM_ERR_ARG, "null matrix");
+  return DISPATCH(h, launch_1q, h, q, R);
+}
+
+int dm_apply_cx(dm_state *h, int control, int target, const double *scale16) {
+  if (int rc = check_handle(h)) return rc;
+  if (control == target || control < 0 || target < 0 || control >= h->n || target >= h->n)
+    return fail(DM_ERR_ARG, "Qubit Labels out of bound in CX Gate (%d, %d; n=%d)", control, target, h->n);
+  return DISPATCH(h, launch_cx, h, control, target, scale16);
+}
+
+int dm_apply_2q(dm_state *h, int qa, int qb, const double R[256]) {
+  if (int rc = check_handle(h)) return rc;
+  if (qa == qb || qa < 0 || qb < 0 || qa >= h->n || qb >= h->n)
+    return fail(DM_ERR_ARG, "bad qubit pair (%d, %d; n=%d)", qa, qb, h->n);
+  if (!R) return fail(DM_ERR_ARG, "null matrix");
+  return DISPATCH(h, launch_2q, h, qa, qb, R);
+}
+
+int dm_memory_noise(dm_state *h, double f, double p, double g) {
+  if (int rc = check_handle(h)) return rc;
+  if (f == 1.0 && g == 1.0) return DM_OK;  // identity (up to the sign of zeros)
+  return DISPATCH(h, launch_memory_noise, h, f, p, g);
+}
+
+int dm_apply_diag(dm_state *h, const double *tab4n) {
+  if (int rc = check_handle(h)) return rc;
+  if (!tab4n) return fail(DM_ERR_ARG, "null table");
+  double tab[4 * kMaxQubits];
+  for (int p = 0; p < h->n; ++p)
+    for (int c = 0; c < 4; ++c) tab[4 * p + c] = tab4n[4 * (h->n - 1 - p) + c];
+  return DISPATCH(h, launch_diag_tab, h, tab, false, 1.0);
+}
+
+int dm_apply_block(dm_state *h, const int *qubits, int k, const double *S) {
+  if (int rc = check_handle(h)) return rc;
+  if (!qubits || !S) return fail(DM_ERR_ARG, "null argument");
+  if (k != 3 && k != 4) return fail(DM_ERR_ARG, "block size k=%d not supported (3 or 4)", k);
+  for (int j = 0; j < k; ++j) {
+    if (int rc = check_qubit(h, qubits[j])) return rc;
+    for (int i = 0; i < j; ++i)
+      if (qubits[i] == qubits[j]) return fail(DM_ERR_ARG, "repeated qubit %d in block", qubits[j]);
+  }
+  const int L = (k == 3) ? 3 : 2;
+  if (h->n < k + L) return fail(DM_ERR_ARG, "block of %d qubits needs n >= %d (n=%d)", k, k + L, h->n);
+  return DISPATCH(h, launch_block, h, qubits, k, S);
+}
+
+int dm_probabilities(dm_state *h, const int *qubits, int m, char basis, const double *nvec, double err,
+                     double *out) {
+  if (int rc = check_handle(h)) return rc;
+  if (m < 1 || m > h->n || !qubits || !out) return fail(DM_ERR_ARG, "bad measured-qubit list");
+  if (basis != 'X' && basis != 'Y' && basis != 'Z' && basis != 'N') return fail(DM_ERR_ARG, "bad basis '%c'", basis);
+  if (basis == 'N' && !nvec) return fail(DM_ERR_ARG, "basis N needs a direction");
+  if (basis == 'N' && m > 13) return fail(DM_ERR_ARG, "basis N gathers 4^m values; m=%d too large", m);
+  for (int t = 0; t < m; ++t) {
+    if (int rc = check_qubit(h, qubits[t])) return rc;
+    if (t && qubits[t] <= qubits[t - 1]) return fail(DM_ERR_ARG, "measured qubits must be ascending");
+  }
+  return DISPATCH(h, do_probabilities, h, qubits, m, basis, nvec, err, out);
+}
+
+int dm_trace_purity(dm_state *h, double *tr, double *purity) {
+  if (int rc = check_handle(h)) return rc;
+  if (tr) {
+    uint64_t zero = 0;
+    double a0 = 0;
+    if (int rc = dm_read_coeffs(h, &zero, 1, &a0)) return rc;
+    *tr = ldexp(a0, h->n);
+  }
+  if (purity) {
+    double s = 0;
+    if (int rc = DISPATCH(h, do_sumsq, h, &s)) return rc;
+    *purity = ldexp(s, h->n);
+  }
+  return DM_OK;
+}
+
+int dm_overlap(dm_state *h, const double *other_host, double *out) {
+  if (int rc = check_handle(h)) return rc;
+  if (!other_host || !out) return fail(DM_ERR_ARG, "null argument");
+  double *d_b = nullptr;
+  HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&d_b), h->N * sizeof(double), h->stream));
+  hipError_t e = hipMemcpyAsync(d_b, other_host, h->N * sizeof(double), hipMemcpyHostToDevice, h->stream);
+  int rc = DM_OK;
+  if (e != hipSuccess) rc = fail(DM_ERR_HIP, "overlap upload failed: %s", hipGetErrorString(e));
+  double s = 0;
+  if (rc == DM_OK) rc = DISPATCH(h, do_dot, h, d_b, &s);
+  hipFreeAsync(d_b, h->stream);
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (rc == DM_OK) *out = ldexp(s, h->n);
+  return rc;
+}
+
+int dm_to_dense(dm_state *h, double *out_c128) {
+  if (int rc = check_handle(h)) return rc;
+  if (!out_c128) return fail(DM_ERR_ARG, "null output");
+  return DISPATCH(h, do_to_dense, h, out_c128);
+}
+
+int dm_run_program(dm_state *h, const dm_op *ops, int nops, const double *params, int64_t nparams) {
+  if (int rc = check_handle(h)) return rc;
+  if (nops < 0 || (nops > 0 && !ops)) return fail(DM_ERR_ARG, "bad op list");
+  if (nparams < 0 || (nparams > 0 && !params)) return fail(DM_ERR_ARG, "bad parameter array");
+  return DISPATCH(h, run_program, h, ops, nops, params, nparams);
+}
+
+int dm_synchronize(dm_state *h) {
+  if (int rc = check_handle(h)) return rc;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return last_launch("stream");
+}
+
+int dm_profile(dm_state *h, int enable) {
+  if (int rc = check_handle(h)) return rc;
+  h->prof = enable != 0;
+  return DM_OK;
+}
+
+int dm_profile_read(dm_state *h, int family, uint64_t *launches, double *ms) {
+  if (int rc = check_handle(h)) return rc;
+  if (family < 0 || family >= kFamilies) return fail(DM_ERR_ARG, "bad kernel family %d", family);
+  if (int rc = resolve_profile(h)) return rc;
+  if (launches) *launches = h->slots[family].launches;
+  if (ms) *ms = h->slots[family].ms;
+  return DM_OK;
+}
+
+int dm_profile_reset(dm_state *h) {
+  if (int rc = check_handle(h)) return rc;
+  if (int rc = resolve_profile(h)) return rc;
+  for (auto &s : h->slots) s = ProfSlot();
+  return DM_OK;
+}
+
+}  // extern "C"

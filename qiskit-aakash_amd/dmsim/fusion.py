@@ -1,0 +1,233 @@
+"""Gate fusion: groups of noisy gates become one k-qubit Pauli-transfer pass.
+
+A noisy gate is a PTM on one or two qubits (the gate followed by its
+channels).  A *block* is a set of gates on at most k qubits that can be
+applied together: its gates keep their order on every qubit, and every gate
+left out of it that comes earlier in the program acts on other qubits only
+(see ``_grow``), so running the block first commutes past them.  A block of one gate runs on the plain kernels;
+blocks on 2 qubits run the dense 16x16 kernel; blocks on 3 or 4 qubits run the
+MFMA block kernel.  ``plan`` chooses the blocks:
+
+* ``greedy``  -- grow each block up to k_max qubits (the classic rule);
+* ``learned`` -- at every step try k = 1..4 and take the block with the most
+  gates per predicted microsecond, the prediction coming from the MLP cost
+  model (dmsim.costmodel); a single gate on its plain kernel is always a
+  candidate, so the plan can never be worse than per-gate application under
+  the model.
+"""
+import collections
+
+import numpy as np
+
+from . import native, ptm
+from .program import Program
+
+NoisyGate = collections.namedtuple('NoisyGate', 'qubits R kind')
+# kind: 'u3' (4x4 R), 'cx' (R = per-pair scale16 or None, signed permutation),
+#       '2q' (16x16 R)
+
+
+def noisy_gates(gates, gate_depolarization=0.0, rotation_error=None, tsp=(1.0, 0.0)):
+    """NoisyGate list from (name, qubits, params) gates: u1/u3 (and u2) as
+    4x4 PTMs followed by the depolarising channel; cx as the signed
+    permutation with the channel as a per-pair scale (or the dense TSP PTM)."""
+    out = []
+    p = float(gate_depolarization)
+    for name, qs, prm in gates:
+        if name in ('u3', 'u1', 'u2'):
+            if name == 'u2':
+                name, prm = 'u3', (np.pi / 2,) + tuple(prm)
+            R = ptm.gate_ptm(name, prm, rotation_error)
+            if p:
+                R = ptm.depolarizing1(p) @ R
+            out.append(NoisyGate((qs[0],), R, 'u3'))
+        elif name == 'cx':
+            if list(tsp) == [1.0, 0.0]:
+                out.append(NoisyGate((qs[0], qs[1]), ptm.depolarizing_pair_scale(p) if p else None, 'cx'))
+            else:
+                R = ptm.cx_ptm(tsp)
+                if p:
+                    R = np.diag(ptm.depolarizing_pair_scale(p)) @ R
+                out.append(NoisyGate((qs[0], qs[1]), R, '2q'))
+        else:
+            raise ValueError('fusion handles u1/u2/u3/cx gates, got %s' % name)
+    return out
+
+
+def gate_ptm16(g):
+    """16x16 PTM (pair index 4*i_q0 + i_q1) of a two-qubit NoisyGate."""
+    if g.kind == '2q':
+        return g.R
+    R = ptm.cx_ptm((1.0, 0.0))
+    if g.R is not None:
+        R = np.diag(g.R) @ R
+    return R
+
+
+def block_ptm(gates, qubits):
+    """4^k x 4^k PTM of the gates (in order) on the block qubits; component
+    index sum_j i_{qubits[j]} 4^j.  Each gate is applied to the rows of the
+    accumulated matrix along its own tensor axes."""
+    k = len(qubits)
+    dim = 4 ** k
+    digit = {q: j for j, q in enumerate(qubits)}
+    M = np.eye(dim).reshape([4] * k + [dim])   # row axes: axis k-1-j is digit j
+    for g in gates:
+        if len(g.qubits) == 1:
+            ax = k - 1 - digit[g.qubits[0]]
+            M = np.moveaxis(np.tensordot(g.R, M, axes=([1], [ax])), 0, ax)
+        else:
+            a0 = k - 1 - digit[g.qubits[0]]
+            a1 = k - 1 - digit[g.qubits[1]]
+            R = gate_ptm16(g).reshape(4, 4, 4, 4)
+            M = np.tensordot(R, M, axes=([2, 3], [a0, a1]))
+            M = np.moveaxis(M, [0, 1], [a0, a1])
+    return np.ascontiguousarray(M.reshape(dim, dim))
+
+
+def _grow(gates, done, start, kmax, n):
+    """Largest block (<= kmax qubits) seeded at the first pending gate
+    ``start``.  Scan forward: a gate joins when none of its qubits is blocked
+    and the union stays within kmax; a gate that cannot join blocks all of its
+    qubits (every later gate on them must run after it)."""
+    qs = []
+    members = []
+    blocked = set()
+    for i in range(start, len(gates)):
+        if done[i]:
+            continue
+        gq = gates[i].qubits
+        if any(q in blocked for q in gq):
+            blocked.update(gq)
+        else:
+            new = [q for q in gq if q not in qs]
+            if len(qs) + len(new) <= kmax:
+                qs.extend(new)
+                members.append(i)
+            else:
+                blocked.update(gq)
+        if qs and all(q in blocked for q in qs) and (len(qs) >= kmax or len(blocked.union(qs)) >= n):
+            break  # nothing further can join
+    return qs, members
+
+
+def min_block_qubits(n):
+    """The MFMA block kernel needs n >= k + column qubits (3+3 or 4+2)."""
+    return 6
+
+
+def plan(gates, n, kmax=4, method='greedy', cost=None):
+    """Blocks as (qubits, [gate indices]) in execution order."""
+    done = [False] * len(gates)
+    blocks = []
+    first = 0
+    allow_mfma = n >= min_block_qubits(n)
+    while True:
+        while first < len(gates) and done[first]:
+            first += 1
+        if first >= len(gates):
+            break
+        if method == 'greedy':
+            km = kmax if allow_mfma else min(kmax, 2)
+            qs, mem = _grow(gates, done, first, km, n)
+        elif method == 'learned':
+            best = None
+            seen = set()
+            for k in range(1, kmax + 1):
+                if k >= 3 and not allow_mfma:
+                    break
+                if k == 1:
+                    qs_k, mem_k = list(gates[first].qubits), [first]
+                else:
+                    qs_k, mem_k = _grow(gates, done, first, k, n)
+                key = tuple(mem_k)
+                if key in seen:
+                    continue
+                seen.add(key)
+                t = cost.block_time(n, qs_k, [gates[i] for i in mem_k])
+                score = len(mem_k) / t
+                if best is None or score > best[2]:
+                    best = (qs_k, mem_k, score)
+            qs, mem = best[0], best[1]
+        else:
+            raise ValueError('unknown fusion method %r' % method)
+        for i in mem:
+            done[i] = True
+        blocks.append((qs, mem))
+    return blocks
+
+
+def add_block(prog, gates, qs, mem):
+    """Append one block to a Program: plain kernels for a single gate, the dense
+    16x16 kernel for a two-qubit block, the MFMA kernel for 3-4 qubits."""
+    if len(mem) == 1:
+        g = gates[mem[0]]
+        if g.kind == 'u3':
+            prog.one_qubit(g.qubits[0], g.R)
+        elif g.kind == 'cx':
+            prog.cx(g.qubits[0], g.qubits[1], g.R)
+        else:
+            prog.two_qubit(g.qubits[0], g.qubits[1], g.R)
+        return
+    sub = [gates[i] for i in mem]
+    if len(qs) == 1:
+        prog.one_qubit(qs[0], block_ptm(sub, qs))
+    elif len(qs) == 2:
+        prog.two_qubit(qs[0], qs[1], block_ptm(sub, qs).reshape(16, 16))
+    else:
+        prog.block(qs, block_ptm(sub, qs))
+
+
+def plain_program(gates, n):
+    """One pass per noisy gate."""
+    prog = Program(n)
+    for i in range(len(gates)):
+        add_block(prog, gates, list(gates[i].qubits), [i])
+    return prog
+
+
+def fused_program(gates, n, kmax=4, method='greedy', cost=None):
+    blocks = plan(gates, n, kmax, method, cost)
+    prog = Program(n)
+    for qs, mem in blocks:
+        add_block(prog, gates, qs, mem)
+    return prog, blocks
+
+
+def _fusable(level):
+    return bool(level) and all(op.name in ('u1', 'u3', 'cx') and getattr(op, 'conditional', None) is None
+                               for op in level)
+
+
+def plan_levels(levels, count, n, backend):
+    """Fused execution of the backend's level loop: maximal runs of levels that
+    hold only u1/u3/cx become one fused program; memory noise between levels
+    enters the gate stream as one 1-qubit channel per qubit and level.
+    Returns {first_clock: (next_clock, runner)}."""
+    f, p, g = backend._decoherence_factor, backend._thermal_factor, backend._decay_factor
+    noisy_levels = not (f == 1 and g == 1)
+    mem_R = ptm.memory_noise(f, p, g)
+    method = backend._fusion
+    cost = None
+    if method == 'learned':
+        from . import costmodel
+        cost = costmodel.default_model()
+    out = {}
+    clock = 0
+    while clock < count:
+        if not _fusable(levels[clock]):
+            clock += 1
+            continue
+        start = clock
+        gates = []
+        while clock < count and _fusable(levels[clock]):
+            seq = [(op.name, tuple(op.qubits), tuple(float(x) for x in getattr(op, 'params', ())))
+                   for op in levels[clock]]
+            gates += noisy_gates(seq, backend._gate_depolarization, backend._rotation_error,
+                                 backend._tsp_model_error)
+            if noisy_levels:
+                gates += [NoisyGate((q,), mem_R, 'u3') for q in range(n)]
+            clock += 1
+        prog, _ = fused_program(gates, n, 4, method, cost)
+        out[start] = (clock, prog.run)
+    return out

@@ -1,0 +1,253 @@
+"""ctypes binding of libdmsim.so (the C ABI declared in include/dmsim.h).
+
+The binding is the only way the Python host touches device state; there is
+no CPU execution path.  Loading fails loudly when the library is missing.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libdmsim.so')
+_lib = None
+_lock = threading.Lock()
+
+FP64, FP32 = 0, 1
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_int_p = ctypes.POINTER(ctypes.c_int)
+c_u64_p = ctypes.POINTER(ctypes.c_uint64)
+
+# name -> (restype, argtypes); every entry point of include/dmsim.h
+SIGNATURES = {
+    'dm_last_error': (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+    'dm_device_info': (ctypes.c_int, [ctypes.c_int, c_int_p, c_int_p, c_u64_p]),
+    'dm_create': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    'dm_destroy': (ctypes.c_int, [ctypes.c_void_p]),
+    'dm_num_qubits': (ctypes.c_int, [ctypes.c_void_p, c_int_p]),
+    'dm_dtype': (ctypes.c_int, [ctypes.c_void_p, c_int_p]),
+    'dm_device_ptr': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    'dm_init_product': (ctypes.c_int, [ctypes.c_void_p, c_double_p, ctypes.c_double]),
+    'dm_upload': (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
+    'dm_download': (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
+    'dm_download_chop': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, c_double_p]),
+    'dm_read_coeffs': (ctypes.c_int, [ctypes.c_void_p, c_u64_p, ctypes.c_int, c_double_p]),
+    'dm_apply_1q': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_double_p]),
+    'dm_apply_cx': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_double_p]),
+    'dm_apply_2q': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_double_p]),
+    'dm_memory_noise': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    'dm_apply_diag': (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
+    'dm_apply_block': (ctypes.c_int, [ctypes.c_void_p, c_int_p, ctypes.c_int, c_double_p]),
+    'dm_probabilities': (ctypes.c_int, [ctypes.c_void_p, c_int_p, ctypes.c_int, ctypes.c_char,
+                                        c_double_p, ctypes.c_double, c_double_p]),
+    'dm_trace_purity': (ctypes.c_int, [ctypes.c_void_p, c_double_p, c_double_p]),
+    'dm_overlap': (ctypes.c_int, [ctypes.c_void_p, c_double_p, c_double_p]),
+    'dm_to_dense': (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
+    'dm_run_program': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, c_double_p,
+                                      ctypes.c_int64]),
+    'dm_synchronize': (ctypes.c_int, [ctypes.c_void_p]),
+    'dm_profile': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    'dm_profile_read': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_u64_p, c_double_p]),
+    'dm_profile_reset': (ctypes.c_int, [ctypes.c_void_p]),
+}
+
+class DmOp(ctypes.Structure):
+    """struct dm_op of include/dmsim.h."""
+    _fields_ = [('kind', ctypes.c_int32), ('k', ctypes.c_int32), ('qubits', ctypes.c_int32 * 4),
+                ('param', ctypes.c_int64)]
+
+
+OP_1Q, OP_CX, OP_CX_SCALED, OP_2Q, OP_BLOCK, OP_MEMORY_NOISE, OP_DIAG = 1, 2, 3, 4, 5, 6, 7
+
+FAMILIES = {'1q': 0, 'cx': 1, '2q': 2, 'memory_noise': 3, 'diag': 4, 'block3': 5, 'reduce': 6, 'init': 7,
+            'block4': 8}
+
+
+class NativeError(RuntimeError):
+    """A failure reported by libdmsim (message from dm_last_error)."""
+
+
+def library_path():
+    return _LIB_PATH
+
+
+def load():
+    """Load libdmsim.so and declare its signatures (idempotent)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(_LIB_PATH):
+            raise NativeError('libdmsim.so not built (expected at %s); run __graft_entry__.build()' % _LIB_PATH)
+        lib = ctypes.CDLL(_LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(512)
+    load().dm_last_error(buf, len(buf))
+    return buf.value.decode(errors='replace')
+
+
+def check(rc, what):
+    if rc != 0:
+        raise NativeError('%s failed (%d): %s' % (what, rc, last_error()))
+
+
+def _dp(arr):
+    return arr.ctypes.data_as(c_double_p)
+
+
+def _f64(x, size=None):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    if size is not None and a.size != size:
+        raise ValueError('expected %d values, got %d' % (size, a.size))
+    return a
+
+
+def device_info(device=0):
+    lib = load()
+    nd, cu, hbm = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_uint64(0)
+    check(lib.dm_device_info(device, ctypes.byref(nd), ctypes.byref(cu), ctypes.byref(hbm)), 'dm_device_info')
+    return {'devices': nd.value, 'cu_count': cu.value, 'hbm_bytes': hbm.value}
+
+
+class DeviceState:
+    """An n-qubit Pauli-coefficient vector resident on the GPU."""
+
+    def __init__(self, n_qubits, precision='fp64'):
+        self.lib = load()
+        self.n = int(n_qubits)
+        self.dtype = FP32 if precision in ('fp32', FP32, 'float32') else FP64
+        h = ctypes.c_void_p()
+        check(self.lib.dm_create(self.n, self.dtype, ctypes.byref(h)), 'dm_create(%d qubits)' % self.n)
+        self.h = h
+
+    # lifecycle ------------------------------------------------------------
+    def close(self):
+        if getattr(self, 'h', None):
+            self.lib.dm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def size(self):
+        return 4 ** self.n
+
+    # state I/O --------------------------------------------------------------
+    def init_product(self, per_qubit4, scale=1.0):
+        v = _f64(per_qubit4, 4 * self.n)
+        check(self.lib.dm_init_product(self.h, _dp(v), float(scale)), 'dm_init_product')
+
+    def upload(self, coeffs):
+        a = _f64(coeffs, self.size)
+        check(self.lib.dm_upload(self.h, _dp(a)), 'dm_upload')
+
+    def download(self, chop=None, out=None):
+        if out is None:
+            out = np.empty(self.size, dtype=np.float64)
+        if chop:
+            check(self.lib.dm_download_chop(self.h, float(chop), _dp(out)), 'dm_download_chop')
+        else:
+            check(self.lib.dm_download(self.h, _dp(out)), 'dm_download')
+        return out
+
+    def read(self, indices):
+        idx = np.ascontiguousarray(np.asarray(indices, dtype=np.uint64))
+        out = np.empty(idx.size, dtype=np.float64)
+        check(self.lib.dm_read_coeffs(self.h, idx.ctypes.data_as(c_u64_p), idx.size, _dp(out)), 'dm_read_coeffs')
+        return out
+
+    # updates ----------------------------------------------------------------
+    def apply_1q(self, q, rows3x4):
+        R = _f64(rows3x4, 12)
+        check(self.lib.dm_apply_1q(self.h, int(q), _dp(R)), 'dm_apply_1q')
+
+    def apply_cx(self, control, target, scale16=None):
+        if scale16 is None:
+            check(self.lib.dm_apply_cx(self.h, int(control), int(target), None), 'dm_apply_cx')
+        else:
+            s = _f64(scale16, 16)
+            check(self.lib.dm_apply_cx(self.h, int(control), int(target), _dp(s)), 'dm_apply_cx')
+
+    def apply_2q(self, qa, qb, R16x16):
+        R = _f64(R16x16, 256)
+        check(self.lib.dm_apply_2q(self.h, int(qa), int(qb), _dp(R)), 'dm_apply_2q')
+
+    def memory_noise(self, f, p, g):
+        check(self.lib.dm_memory_noise(self.h, float(f), float(p), float(g)), 'dm_memory_noise')
+
+    def apply_diag(self, tab_nx4):
+        t = _f64(tab_nx4, 4 * self.n)
+        check(self.lib.dm_apply_diag(self.h, _dp(t)), 'dm_apply_diag')
+
+    def apply_block(self, qubits, S):
+        qs = np.ascontiguousarray(np.asarray(qubits, dtype=np.int32))
+        k = qs.size
+        M = _f64(S, 4 ** (2 * k))
+        check(self.lib.dm_apply_block(self.h, qs.ctypes.data_as(c_int_p), k, _dp(M)), 'dm_apply_block')
+
+    # reductions -------------------------------------------------------------
+    def probabilities(self, qubits, basis='Z', err=1.0, nvec=None):
+        qs = np.ascontiguousarray(np.asarray(qubits, dtype=np.int32))
+        out = np.empty(2 ** qs.size, dtype=np.float64)
+        nv = _f64(nvec if nvec is not None else [0.0, 0.0, 1.0], 3)
+        check(self.lib.dm_probabilities(self.h, qs.ctypes.data_as(c_int_p), qs.size, basis.encode()[:1],
+                                        _dp(nv), float(err), _dp(out)), 'dm_probabilities')
+        return out
+
+    def trace_purity(self):
+        tr, pur = ctypes.c_double(0), ctypes.c_double(0)
+        check(self.lib.dm_trace_purity(self.h, ctypes.byref(tr), ctypes.byref(pur)), 'dm_trace_purity')
+        return tr.value, pur.value
+
+    def overlap(self, other_coeffs):
+        b = _f64(other_coeffs, self.size)
+        out = ctypes.c_double(0)
+        check(self.lib.dm_overlap(self.h, _dp(b), ctypes.byref(out)), 'dm_overlap')
+        return out.value
+
+    def to_dense(self):
+        dim = 2 ** self.n
+        out = np.empty((dim, dim), dtype=np.complex128)
+        check(self.lib.dm_to_dense(self.h, out.ctypes.data_as(c_double_p)), 'dm_to_dense')
+        return out
+
+    def run_program(self, ops, params):
+        """``ops``: ctypes array of DmOp; ``params``: float64 array."""
+        pv = _f64(params) if params is not None and len(params) else np.zeros(1)
+        check(self.lib.dm_run_program(self.h, ctypes.cast(ops, ctypes.c_void_p), len(ops), _dp(pv),
+                                      int(len(params) if params is not None else 0)), 'dm_run_program')
+
+    def synchronize(self):
+        check(self.lib.dm_synchronize(self.h), 'dm_synchronize')
+
+    # timing -----------------------------------------------------------------
+    def profile(self, enable=True):
+        check(self.lib.dm_profile(self.h, 1 if enable else 0), 'dm_profile')
+
+    def profile_reset(self):
+        check(self.lib.dm_profile_reset(self.h), 'dm_profile_reset')
+
+    def profile_read(self, family):
+        n, ms = ctypes.c_uint64(0), ctypes.c_double(0)
+        check(self.lib.dm_profile_read(self.h, FAMILIES[family], ctypes.byref(n), ctypes.byref(ms)),
+              'dm_profile_read')
+        return n.value, ms.value

@@ -1,0 +1,72 @@
+"""Op lists for ``dm_run_program``: build once, run many times."""
+import numpy as np
+
+from . import native, ptm
+
+
+class Program:
+    """Accumulates operations and their parameters for one register size."""
+
+    def __init__(self, n):
+        self.n = n
+        self._ops = []
+        self._vals = []
+        self._size = 0
+        self._compiled = None
+
+    def __len__(self):
+        return len(self._ops)
+
+    def _add(self, kind, qubits, values=None, k=None):
+        off = self._size
+        if values is not None:
+            v = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(-1))
+            self._vals.append(v)
+            self._size += v.size
+        qs = list(qubits) + [0] * (4 - len(qubits))
+        self._ops.append((kind, k if k is not None else len(qubits), qs, off))
+        self._compiled = None
+
+    def one_qubit(self, q, R4):
+        self._add(native.OP_1Q, [q], ptm.rows3(R4))
+
+    def cx(self, control, target, scale16=None):
+        if scale16 is None:
+            self._add(native.OP_CX, [control, target])
+        else:
+            self._add(native.OP_CX_SCALED, [control, target], scale16)
+
+    def two_qubit(self, qa, qb, R16):
+        self._add(native.OP_2Q, [qa, qb], R16)
+
+    def block(self, qubits, S):
+        self._add(native.OP_BLOCK, list(qubits), S, k=len(qubits))
+
+    def memory_noise(self, f, p, g):
+        self._add(native.OP_MEMORY_NOISE, [], [f, p, g])
+
+    def diag(self, tab_nx4):
+        self._add(native.OP_DIAG, [], tab_nx4)
+
+    def kinds(self):
+        return [o[0] for o in self._ops]
+
+    def compile(self):
+        if self._compiled is None:
+            arr = (native.DmOp * max(1, len(self._ops)))()
+            for i, (kind, k, qs, off) in enumerate(self._ops):
+                arr[i].kind = kind
+                arr[i].k = k
+                for j in range(4):
+                    arr[i].qubits[j] = qs[j]
+                arr[i].param = off
+            params = np.concatenate(self._vals) if self._vals else np.zeros(0)
+            self._compiled = (arr, params)
+        return self._compiled
+
+    def run(self, state):
+        if not self._ops:
+            return
+        arr, params = self.compile()
+        ops = (native.DmOp * len(self._ops)).from_buffer(arr)
+        state.run_program(ops, params)

@@ -1,0 +1,258 @@
+"""Kernel-level parity: each C-ABI entry point against the CPU oracle on
+seeded random states, at every qubit position (strides 1, 4, 16, ...).
+
+FP64: max |delta a| <= 1e-14 (single operations; the end-to-end bar is 1e-12).
+FP32: max |delta a| <= 2e-7 * 2^-n * ... (relative to the coefficient scale)
+and the identity coefficient a_0 = 2^-n stays exact.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import refsim
+from dmsim import native, ptm
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_state(n, rng):
+    """Random physical-looking Pauli vector: a_0 = 2^-n, others ~ 2^-n/ sqrt scale."""
+    a = rng.normal(size=4 ** n) * (0.3 * 2.0 ** -n)
+    a[0] = 2.0 ** -n
+    return a
+
+
+def tol(prec, n):
+    return 1e-14 * 2.0 ** -n * 8 if prec == 'fp64' else 3e-7 * 2.0 ** -n * 8
+
+
+def make(n, prec, a):
+    st = native.DeviceState(n, prec)
+    st.upload(a)
+    return st
+
+
+PRECS = ['fp64', 'fp32']
+
+
+@pytest.mark.parametrize('prec', PRECS)
+@pytest.mark.parametrize('n', [1, 2, 4, 7])
+def test_u3_all_positions(prec, n):
+    rng = np.random.default_rng(100 + n)
+    for q in range(n):
+        a = rand_state(n, rng)
+        th, ph, la = rng.uniform(0, 2 * np.pi, 3)
+        st = make(n, prec, a)
+        st.apply_1q(q, ptm.rows3(ptm.gate_ptm('u3', [th, ph, la])))
+        got = st.download()
+        want = a.reshape(n * [4]).copy()
+        st3 = want.reshape(4 ** q, 4, 4 ** (n - q - 1))
+        for axis, ang in refsim.gate_rotations('u3', [th, ph, la]):
+            st3 = refsim.rotate_components(st3, axis, ang, [1.0, 0.0])
+        assert np.max(np.abs(got - st3.reshape(-1))) <= tol(prec, n), (q, prec)
+        assert got[0] == 2.0 ** -n
+        st.close()
+
+
+@pytest.mark.parametrize('prec', PRECS)
+def test_1q_affine_reset_and_collapse(prec):
+    n = 5
+    rng = np.random.default_rng(7)
+    for q in range(n):
+        a = rand_state(n, rng)
+        st = make(n, prec, a)
+        st.apply_1q(q, ptm.rows3(ptm.reset()))
+        want = refsim.reset_qubit(a.reshape(n * [4]).copy(), n, q).reshape(-1)
+        assert np.max(np.abs(st.download() - want)) <= tol(prec, n)
+        st.apply_1q(q, ptm.rows3(ptm.collapse_axis(2, 0.9)))
+        want, _ = refsim.collapse_axis(want.reshape(n * [4]).copy(), n, q, 2, 0.9)
+        assert np.max(np.abs(st.download() - want.reshape(-1))) <= tol(prec, n)
+        st.close()
+
+
+@pytest.mark.parametrize('prec', PRECS)
+@pytest.mark.parametrize('n', [2, 3, 5])
+def test_cx_all_pairs(prec, n):
+    rng = np.random.default_rng(200 + n)
+    for c, t in itertools.permutations(range(n), 2):
+        a = rand_state(n, rng)
+        st = make(n, prec, a)
+        st.apply_cx(c, t)
+        want = refsim.apply_cx(a.reshape(n * [4]).copy(), c, t, [1.0, 0.0], n).reshape(-1)
+        assert np.max(np.abs(st.download() - want)) <= tol(prec, n), (c, t)
+        st.close()
+
+
+@pytest.mark.parametrize('prec', PRECS)
+def test_cx_with_depolarizing_scale(prec):
+    n = 4
+    rng = np.random.default_rng(3)
+    for c, t in [(0, 3), (3, 0), (1, 2), (2, 3)]:
+        a = rand_state(n, rng)
+        st = make(n, prec, a)
+        st.apply_cx(c, t, ptm.depolarizing_pair_scale(0.01))
+        want = refsim.apply_cx(a.reshape(n * [4]).copy(), c, t, [1.0, 0.0], n)
+        want = refsim.apply_gate_depolarizing(want, n, [c, t], 0.01).reshape(-1)
+        assert np.max(np.abs(st.download() - want)) <= tol(prec, n)
+        st.close()
+
+
+@pytest.mark.parametrize('prec', PRECS)
+def test_tsp_cx_dense(prec):
+    n = 4
+    rng = np.random.default_rng(4)
+    tsp = [0.97, 0.05]
+    for c, t in [(0, 1), (1, 0), (0, 3), (3, 1)]:
+        a = rand_state(n, rng)
+        st = make(n, prec, a)
+        st.apply_2q(c, t, ptm.cx_ptm(tsp))
+        want = refsim.apply_cx(a.reshape(n * [4]).copy(), c, t, tsp, n).reshape(-1)
+        assert np.max(np.abs(st.download() - want)) <= tol(prec, n)
+        st.close()
+
+
+@pytest.mark.parametrize('prec', PRECS)
+@pytest.mark.parametrize('fpg', [(0.9, 0.5, 0.99), (0.95, 1.0, 0.97), (0.8, 0.0, 0.9)])
+@pytest.mark.parametrize('n', [1, 2, 4, 7])
+def test_memory_noise(prec, fpg, n):
+    rng = np.random.default_rng(300 + n)
+    a = rand_state(n, rng)
+    st = make(n, prec, a)
+    st.memory_noise(*fpg)
+    f, p, g = fpg
+    want = refsim.apply_memory_noise(a.reshape(n * [4]).copy(), n, f, p, g).reshape(-1)
+    assert np.max(np.abs(st.download() - want)) <= tol(prec, n) * 4
+    st.close()
+
+
+@pytest.mark.parametrize('prec', PRECS)
+def test_diag_and_init_product(prec):
+    n = 6
+    rng = np.random.default_rng(5)
+    vecs = rng.normal(size=(n, 4))
+    st = native.DeviceState(n, prec)
+    st.init_product(vecs.reshape(-1), 0.5 ** n)
+    want = np.array([1.0])
+    for q in range(n):
+        want = np.kron(want, vecs[q])
+    want *= 0.5 ** n
+    rel = 1e-15 if prec == 'fp64' else 1e-6
+    assert np.max(np.abs(st.download() - want)) <= rel * np.max(np.abs(want)) * 4
+    tab = rng.uniform(0.5, 1.0, size=(n, 4))
+    st.apply_diag(tab.reshape(-1))
+    fac = np.array([1.0])
+    for q in range(n):
+        fac = np.kron(fac, tab[q])
+    assert np.max(np.abs(st.download() - want * fac)) <= rel * np.max(np.abs(want)) * 8
+    st.close()
+
+
+@pytest.mark.parametrize('prec', PRECS)
+@pytest.mark.parametrize('basis', ['X', 'Y', 'Z'])
+def test_probabilities_ensemble_and_partial(prec, basis):
+    n = 6
+    rng = np.random.default_rng(ord(basis))
+    a = rand_state(n, rng)
+    st = make(n, prec, a)
+    err = 0.93
+    got = st.probabilities(list(range(n)), basis, err)
+    want = np.array(list(refsim.ensemble_probabilities(a.reshape(n * [4]), n, basis, None, err).values()))
+    t = 1e-13 if prec == 'fp64' else 1e-5
+    assert np.max(np.abs(got - want)) <= t
+    meas = [1, 4]
+    got = st.probabilities(meas, basis, err)
+    full = want.reshape(n * [2])
+    rest = tuple(q for q in range(n) if q not in meas)
+    assert np.max(np.abs(got - full.sum(axis=rest).reshape(-1))) <= t
+    st.close()
+
+
+def test_probabilities_n_basis():
+    n = 4
+    rng = np.random.default_rng(8)
+    a = rand_state(n, rng)
+    st = make(n, 'fp64', a)
+    nv = np.array([1.0, 2.0, 3.0]) / np.sqrt(14)
+    got = st.probabilities(list(range(n)), 'N', 0.9, nv)
+    want = np.array(list(refsim.ensemble_probabilities(a.reshape(n * [4]), n, 'N', nv, 0.9).values()))
+    assert np.max(np.abs(got - want)) <= 1e-14
+    st.close()
+
+
+@pytest.mark.parametrize('prec', PRECS)
+def test_trace_purity_overlap(prec):
+    n = 7
+    rng = np.random.default_rng(9)
+    a = rand_state(n, rng)
+    b = rand_state(n, rng)
+    st = make(n, prec, a)
+    tr, pur = st.trace_purity()
+    assert tr == 1.0
+    t = 1e-12 if prec == 'fp64' else 1e-5
+    assert abs(pur - 2 ** n * np.dot(a, a)) <= t * abs(pur)
+    assert abs(st.overlap(b) - 2 ** n * np.dot(a, b)) <= t
+    st.close()
+
+
+@pytest.mark.parametrize('n', [1, 3, 5])
+def test_to_dense_matches_oracle_bitwise(n):
+    rng = np.random.default_rng(10 + n)
+    a = rand_state(n, rng)
+    st = make(n, 'fp64', a)
+    got = st.to_dense()
+    want = refsim.pauli_to_matrix(a, n)
+    assert np.array_equal(got, want)
+    st.close()
+
+
+def test_download_chop():
+    n = 3
+    a = np.zeros(64)
+    a[0] = 0.125
+    a[5] = 3e-16
+    a[7] = -2e-15
+    st = make(n, 'fp64', a)
+    got = st.download(chop=1e-15)
+    assert got[5] == 0.0 and got[7] == -2e-15 and got[0] == 0.125
+    st.close()
+
+
+def test_errors_are_reported():
+    st = native.DeviceState(3, 'fp64')
+    with pytest.raises(native.NativeError, match='out of range'):
+        st.apply_1q(3, np.zeros(12))
+    with pytest.raises(native.NativeError, match='CX'):
+        st.apply_cx(1, 1)
+    st.close()
+
+
+def apply_block_np(a, n, qubits, S):
+    """numpy contraction of a k-qubit PTM (component index sum_j i_{qubits[j]} 4^j)."""
+    k = len(qubits)
+    A = a.reshape(n * [4])
+    St = np.asarray(S).reshape([4] * (2 * k))  # [out_{k-1}..out_0, in_{k-1}..in_0]
+    in_axes = list(range(k, 2 * k))
+    # tensor axis k+m of St is input digit k-1-m -> qubit qubits[k-1-m]
+    qaxes = [qubits[k - 1 - m] for m in range(k)]
+    out = np.tensordot(St, A, axes=(in_axes, qaxes))  # [out digits..., remaining qubit axes]
+    rem = [q for q in range(n) if q not in qaxes]
+    # current axes: out_{k-1}..out_0 (-> qubits qaxes), then rem
+    cur = qaxes + rem
+    perm = [cur.index(q) for q in range(n)]
+    return np.transpose(out, perm).reshape(-1)
+
+
+@pytest.mark.parametrize('prec', PRECS)
+@pytest.mark.parametrize('k,n,qubits', [(3, 6, [0, 1, 2]), (3, 7, [6, 2, 4]), (3, 8, [1, 7, 3]),
+                                        (4, 6, [5, 0, 3, 2]), (4, 8, [2, 5, 6, 7]), (4, 7, [0, 1, 2, 3])])
+def test_block_mfma_general_matrix(prec, k, n, qubits):
+    rng = np.random.default_rng(17 * k + n)
+    a = rand_state(n, rng)
+    S = rng.normal(size=(4 ** k, 4 ** k)) / 4 ** k
+    st = make(n, prec, a)
+    st.apply_block(qubits, S)
+    want = apply_block_np(a, n, qubits, S)
+    t = 1e-15 * 2.0 ** -n * 4 ** k if prec == 'fp64' else 2e-6 * 2.0 ** -n * 4
+    assert np.max(np.abs(st.download() - want)) <= t
+    st.close()
