@@ -75,6 +75,12 @@ def kernel_roofline(fams, n, precision):
     dom = max(fams, key=lambda f: fams[f][1])
     launches, ms = fams[dom]
     avg_s = ms / 1e3 / max(1, launches)
+    if dom == 'fused':
+        # a tile pass reads and writes every coefficient once
+        byts = 2.0 * N * s
+        ach = byts / avg_s / 1e9
+        return dom, {'bound': 'hbm', 'achieved': ach, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': ach / HBM_PEAK_GBS, 'traffic': None}, avg_s, byts, 0.0
     if dom in ('block3', 'block4'):
         k = 3 if dom == 'block3' else 4
         flops = 2.0 * 4 ** k * N
@@ -180,6 +186,7 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-accuracy', action='store_true')
     ap.add_argument('--fusion', default='auto', choices=['auto', 'greedy', 'learned'])
+    ap.add_argument('--quick', action='store_true', help='time only the plain fp64 and headline paths')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU: the path has no multi-GPU form in this build (--gpus %d refused)'
@@ -203,7 +210,7 @@ def main():
             log('no learned cost model (%s); greedy fusion' % e)
             method = 'greedy'
     t0 = time.perf_counter()
-    fused, blocks = fusion.fused_program(noisy, n, 4, method, cost)
+    fused, blocks = fusion.fused_program(noisy, n, 4, method, cost, 'seq')
     plan_s = time.perf_counter() - t0
     plain = fusion.plain_program(noisy, n)
     ks = [len(q) if len(m) > 1 else 0 for q, m in blocks]
@@ -216,9 +223,18 @@ def main():
     log('timing fused fp32 ...')
     t_fused, f_fused = time_program(fused, n, 'fp32', args.steps, args.warmup)
     log('  %.3f s for %d steps' % (t_fused, args.steps))
-    log('timing plain fp32 / fused fp64 ...')
-    t_plain32, f_plain32 = time_program(plain, n, 'fp32', args.steps, args.warmup)
-    t_fused64, _ = time_program(fused, n, 'fp64', args.steps, args.warmup)
+    extra = {}
+    if not args.quick:
+        log('timing plain fp32, fused fp64, greedy dense-MFMA fp32 ...')
+        extra['plain_fp32'] = time_program(plain, n, 'fp32', args.steps, args.warmup)
+        extra['fused_fp64'] = time_program(fused, n, 'fp64', args.steps, args.warmup)
+        dense, _ = fusion.fused_program(noisy, n, 4, 'greedy', None, 'dense')
+        extra['greedy_dense_fp32'] = time_program(dense, n, 'fp32', args.steps, args.warmup)
+        dense3, _ = fusion.fused_program(noisy, n, 3, 'greedy', None, 'dense')
+        extra['greedy_dense_k3_fp32'] = time_program(dense3, n, 'fp32', args.steps, args.warmup)
+        if method != 'greedy':
+            gseq, _ = fusion.fused_program(noisy, n, 4, 'greedy', None, 'seq')
+            extra['greedy_seq_fp32'] = time_program(gseq, n, 'fp32', args.steps, args.warmup)
 
     gps = lambda t: G * args.steps / t
     value = gps(t_fused)
@@ -250,13 +266,13 @@ def main():
             'path': 'fused fp32 (%s partition, %d passes)' % (method, len(fused)),
             'fusion': method,
             'plain_fp64_gates_per_s': gps(t_plain),
-            'plain_fp32_gates_per_s': gps(t_plain32),
-            'fused_fp64_gates_per_s': gps(t_fused64),
+            'other_paths_gates_per_s': {k: gps(v[0]) for k, v in extra.items()},
             'speedup_fused_fp32_vs_plain_fp64': value / gps(t_plain),
             'plain_fp64_dominant_kernel': pdom,
             'plain_fp64_roofline': proof,
             'plain_fp64_streaming_hbm_frac': streaming_fraction(f_plain, n, 'fp64', None),
-            'plain_fp32_streaming_hbm_frac': streaming_fraction(f_plain32, n, 'fp32', None),
+            'plain_fp32_streaming_hbm_frac': (streaming_fraction(extra['plain_fp32'][1], n, 'fp32', None)
+                                              if 'plain_fp32' in extra else None),
             'fused_fp32_kernel_ms': {k: v[1] / max(1, v[0]) for k, v in f_fused.items() if v[0]},
             'fused_fp32_launches': {k: v[0] for k, v in f_fused.items() if v[0]},
             'plan_host_s': plan_s,

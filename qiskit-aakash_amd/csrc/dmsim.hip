@@ -564,6 +564,186 @@ __global__ __launch_bounds__(kBlock) void k_block_mfma(T *__restrict__ a, const 
 }
 
 // ---------------------------------------------------------------------------
+// K5': tile program.  One workgroup stages a tile of 4^M coefficients -- the
+// digits at positions tpos[0..M-1] (ascending; tpos[j] == j for j < w, so
+// runs of 4^w coefficients are contiguous in HBM) with the other positions
+// fixed -- in LDS, applies a short list of operations to it and writes it
+// back: one HBM round trip for a whole group of gates.  Operations address
+// tile digits:
+//   TOP_1Q     digit d0, 12 values (rows X,Y,Z of a 4x4 PTM)
+//   TOP_CX     control digit d0, target digit d1, k=1: 16 per-pair scales
+//   TOP_2Q     digits d0, d1, 256 values (16x16 PTM, pair index 4 i_d0 + i_d1)
+//   TOP_DENSE  k = 3 or 4 digits d0..d{k-1}: 4^k x 4^k PTM on the matrix
+//              cores, values stored transposed (ST[col][row]); the other
+//              tile digits index the 4^(M-k) columns.
+// ---------------------------------------------------------------------------
+
+enum { TOP_1Q = 1, TOP_CX = 2, TOP_2Q = 3, TOP_DENSE = 4 };
+
+struct TileOp {
+  int32_t kind, k, d[4], param, pad;
+};
+
+struct TileGeom {
+  int m;                  // tile digits (== template M)
+  int w;                  // leading digits with tpos[j] == j
+  int tpos[8];
+  int nr;
+  int rpos[kMaxQubits];
+  uint64_t tiles;
+};
+
+__device__ __forceinline__ int ins_zero(int g, int j) {
+  const int lo = g & ((1 << (2 * j)) - 1);
+  return ((g >> (2 * j)) << (2 * j + 2)) | lo;
+}
+
+template <typename T, int M>
+__device__ __forceinline__ void tile_dense(T *s, const TileOp &op, const T *__restrict__ ST) {
+  // rows: digits op.d[0..k-1] (row digit j -> tile digit d[j]); cols: the
+  // remaining tile digits in ascending order.
+  using MT = MfmaT<T>;
+  using Acc = typename MT::acc;
+  const int k = op.k;
+  const int rows = 1 << (2 * k);
+  const int cols = 1 << (2 * (M - k));
+  int cdig[8];
+  {
+    int c = 0;
+    for (int j = 0; j < M; ++j) {
+      bool inrow = false;
+      for (int r = 0; r < k; ++r) inrow = inrow || (op.d[r] == j);
+      if (!inrow) cdig[c++] = j;
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int CB = cols / 16;
+  const int NB = (rows / 16) * CB;
+  const int per_wave = NB / 4;  // 4 for (M,k) = (6,3), (6,4)
+  Acc acc[4];
+  int rb_[4], cb_[4], coff[4];
+  for (int i = 0; i < 4; ++i) {
+    acc[i] = Acc{0, 0, 0, 0};
+    const int ob = wave * per_wave + i;
+    rb_[i] = ob / CB;
+    cb_[i] = ob % CB;
+    const int col = cb_[i] * 16 + lr;
+    int o = 0;
+    for (int j = 0; j < M - k; ++j) o |= ((col >> (2 * j)) & 3) << (2 * cdig[j]);
+    coff[i] = o;
+  }
+  for (int k0 = 0; k0 < rows; k0 += 4) {
+    const int kk = k0 + lk;
+    int roff = 0;
+    for (int j = 0; j < k; ++j) roff |= ((kk >> (2 * j)) & 3) << (2 * op.d[j]);
+    for (int i = 0; i < per_wave; ++i) {
+      const T av = ST[(size_t)kk * rows + rb_[i] * 16 + lr];
+      const T bv = s[roff + coff[i]];
+      acc[i] = MT::mma(av, bv, acc[i]);
+    }
+  }
+  __syncthreads();  // every wave has read its B fragments
+  for (int i = 0; i < per_wave; ++i) {
+    for (int v = 0; v < 4; ++v) {
+      const int row = rb_[i] * 16 + MT::row(lk, v);
+      int roff = 0;
+      for (int j = 0; j < k; ++j) roff |= ((row >> (2 * j)) & 3) << (2 * op.d[j]);
+      s[roff + coff[i]] = acc[i][v];
+    }
+  }
+}
+
+template <typename T, int M>
+__global__ __launch_bounds__(kBlock) void k_tile_prog(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
+                                                      int nops, const T *__restrict__ prm) {
+  constexpr int E = 1 << (2 * M);
+  __shared__ __align__(32) T s[E];
+  const int tid = threadIdx.x;
+  const int w = geo.w;
+  const int lomask = (1 << (2 * w)) - 1;
+  for (uint64_t tile = blockIdx.x; tile < geo.tiles; tile += gridDim.x) {
+    uint64_t base = 0;
+    {
+      uint64_t r = tile;
+      for (int j = 0; j < geo.nr; ++j) {
+        base |= (r & 3ull) << (2 * geo.rpos[j]);
+        r >>= 2;
+      }
+    }
+    // stage: quads of 4 consecutive tile indices are 4 consecutive coefficients
+    for (int c = tid; c < E / 4; c += kBlock) {
+      const int t = 4 * c;
+      uint64_t off = (uint64_t)(t & lomask);
+      for (int j = w; j < M; ++j) off |= (uint64_t)((t >> (2 * j)) & 3) << (2 * geo.tpos[j]);
+      *reinterpret_cast<Pack<T, 4> *>(&s[t]) = ldp<Pack<T, 4>>(a + base + off);
+    }
+    __syncthreads();
+    for (int i = 0; i < nops; ++i) {
+      const TileOp op = ops[i];
+      const T *pv = prm + op.param;
+      if (op.kind == TOP_1Q) {
+        const int j = op.d[0];
+        const int st = 1 << (2 * j);
+        T r[3][4];
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+          for (int y = 0; y < 4; ++y) r[x][y] = pv[4 * x + y];
+        for (int g = tid; g < E / 4; g += kBlock) {
+          const int b = ins_zero(g, j);
+          const T x0 = s[b], x1 = s[b + st], x2 = s[b + 2 * st], x3 = s[b + 3 * st];
+          s[b + st] = r[0][0] * x0 + r[0][1] * x1 + r[0][2] * x2 + r[0][3] * x3;
+          s[b + 2 * st] = r[1][0] * x0 + r[1][1] * x1 + r[1][2] * x2 + r[1][3] * x3;
+          s[b + 3 * st] = r[2][0] * x0 + r[2][1] * x1 + r[2][2] * x2 + r[2][3] * x3;
+        }
+      } else if (op.kind == TOP_CX || op.kind == TOP_2Q) {
+        const int jc = op.d[0], jt = op.d[1];
+        const int jl = min(jc, jt), jh = max(jc, jt);
+        const int sc = 1 << (2 * jc), stt = 1 << (2 * jt);
+        for (int g = tid; g < E / 16; g += kBlock) {
+          const int b = ins_zero(ins_zero(g, jl), jh);
+          T v[4][4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) v[c][t] = s[b + c * sc + t * stt];
+          if (op.kind == TOP_CX) {
+            Scale16<T> sv;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) sv.s[q] = op.k ? pv[q] : T(1);
+            cx_map<T, T, true, true>(v, sv);
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+              for (int t = 0; t < 4; ++t)
+                if (c | t) s[b + c * sc + t * stt] = v[c][t];
+          } else {
+#pragma unroll
+            for (int r = 1; r < 16; ++r) {
+              T acc = T(0);
+#pragma unroll
+              for (int c = 0; c < 16; ++c) acc += pv[16 * r + c] * v[c >> 2][c & 3];
+              s[b + (r >> 2) * sc + (r & 3) * stt] = acc;
+            }
+          }
+        }
+      } else if (op.kind == TOP_DENSE) {
+        if (M == 6) tile_dense<T, M>(s, op, pv);
+      }
+      __syncthreads();
+    }
+    for (int c = tid; c < E / 4; c += kBlock) {
+      const int t = 4 * c;
+      uint64_t off = (uint64_t)(t & lomask);
+      for (int j = w; j < M; ++j) off |= (uint64_t)((t >> (2 * j)) & 3) << (2 * geo.tpos[j]);
+      stp(a + base + off, *reinterpret_cast<const Pack<T, 4> *>(&s[t]));
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // K6: reductions (FP64 accumulation).
 // ---------------------------------------------------------------------------
 
@@ -755,6 +935,8 @@ struct dm_state {
   size_t red_cap = 0;           // doubles
   void *d_mat = nullptr;        // block matrices (transposed, storage dtype)
   size_t mat_cap = 0;           // bytes
+  void *d_ops = nullptr;        // tile-program operations
+  size_t ops_cap = 0;           // bytes
   double *d_aux = nullptr;      // gathered vectors / overlap operand
   size_t aux_cap = 0;           // bytes
   // profiling
@@ -1221,6 +1403,7 @@ int param_count(const dm_state *h, const dm_op &op) {
     case DM_OP_BLOCK: return 1 << (4 * op.k);
     case DM_OP_MEMORY_NOISE: return 3;
     case DM_OP_DIAG: return 4 * h->n;
+    case DM_OP_FUSED: return 0;  // variable length, checked by the parser
     default: return -1;
   }
 }
@@ -1251,29 +1434,177 @@ int validate_op(const dm_state *h, const dm_op &op, int i, int64_t nparams) {
   return DM_OK;
 }
 
+// A fused pass parsed from its parameter record (see DM_OP_FUSED in dmsim.h).
+struct FusedPass {
+  int m = 0;
+  TileGeom geo;
+  std::vector<TileOp> ops;     // param offsets relative to this pass's values
+  std::vector<double> vals;    // values in kernel layout (FP64; converted at upload)
+};
+
+int parse_fused(const dm_state *h, const double *p, int64_t avail, int opi, FusedPass *fp) {
+  int64_t at = 0;
+  auto next = [&](double *out) -> bool {
+    if (at >= avail) return false;
+    *out = p[at++];
+    return true;
+  };
+  auto next_int = [&](int *out) -> bool {
+    double v;
+    if (!next(&v) || v != (double)(int)v) return false;
+    *out = (int)v;
+    return true;
+  };
+  int m = 0, nq = 0;
+  if (!next_int(&m) || (m != 5 && m != 6)) return fail(DM_ERR_ARG, "op %d: fused tile size must be 5 or 6", opi);
+  if (h->n < m) return fail(DM_ERR_ARG, "op %d: fused tile of %d qubits on %d qubits", opi, m, h->n);
+  if (!next_int(&nq) || nq < 0 || nq > m) return fail(DM_ERR_ARG, "op %d: bad fused qubit count", opi);
+  bool intile[kMaxQubits] = {false};
+  int count = 0;
+  for (int i = 0; i < nq; ++i) {
+    int q;
+    if (!next_int(&q) || q < 0 || q >= h->n) return fail(DM_ERR_ARG, "op %d: bad fused qubit", opi);
+    const int pos = h->n - 1 - q;
+    if (!intile[pos]) {
+      intile[pos] = true;
+      ++count;
+    }
+  }
+  for (int pos = 0; pos < h->n && count < m; ++pos)
+    if (!intile[pos]) {
+      intile[pos] = true;
+      ++count;
+    }
+  if (!intile[0]) return fail(DM_ERR_ARG, "op %d: fused tile must hold the last qubit's position", opi);
+  memset(&fp->geo, 0, sizeof(fp->geo));
+  fp->m = m;
+  fp->geo.m = m;
+  int digit_of_pos[kMaxQubits];
+  int j = 0;
+  for (int pos = 0; pos < h->n; ++pos)
+    if (intile[pos]) {
+      digit_of_pos[pos] = j;
+      fp->geo.tpos[j++] = pos;
+    } else {
+      fp->geo.rpos[fp->geo.nr++] = pos;
+    }
+  int w = 0;
+  while (w < m && fp->geo.tpos[w] == w) ++w;
+  fp->geo.w = w;
+  fp->geo.tiles = 1ull << (2 * fp->geo.nr);
+  int nsub = 0;
+  if (!next_int(&nsub) || nsub < 0) return fail(DM_ERR_ARG, "op %d: bad fused op count", opi);
+  for (int i = 0; i < nsub; ++i) {
+    int kind, kq, nv, qs[4];
+    if (!next_int(&kind) || !next_int(&kq) || kq < 0 || kq > 4)
+      return fail(DM_ERR_ARG, "op %d.%d: bad fused sub-op header", opi, i);
+    for (int x = 0; x < 4; ++x)
+      if (!next_int(&qs[x])) return fail(DM_ERR_ARG, "op %d.%d: bad fused sub-op qubits", opi, i);
+    if (!next_int(&nv) || nv < 0 || at + nv > avail) return fail(DM_ERR_ARG, "op %d.%d: bad value count", opi, i);
+    TileOp op;
+    memset(&op, 0, sizeof(op));
+    op.kind = kind;
+    op.k = kq;
+    for (int x = 0; x < kq; ++x) {
+      if (qs[x] < 0 || qs[x] >= h->n || !intile[h->n - 1 - qs[x]])
+        return fail(DM_ERR_ARG, "op %d.%d: qubit %d is not in the fused tile", opi, i, qs[x]);
+      op.d[x] = digit_of_pos[h->n - 1 - qs[x]];
+      for (int y = 0; y < x; ++y)
+        if (qs[y] == qs[x]) return fail(DM_ERR_ARG, "op %d.%d: repeated qubit", opi, i);
+    }
+    int want = -1;
+    if (kind == TOP_1Q && kq == 1) want = 12;
+    else if (kind == TOP_CX && kq == 2) want = (nv == 16) ? 16 : 0;
+    else if (kind == TOP_2Q && kq == 2) want = 256;
+    else if (kind == TOP_DENSE && (kq == 3 || kq == 4) && m == 6) want = 1 << (4 * kq);
+    if (want < 0 || nv != want) return fail(DM_ERR_ARG, "op %d.%d: bad fused sub-op (kind %d, %d qubits, %d values)",
+                                           opi, i, kind, kq, nv);
+    if (kind == TOP_CX) op.k = (nv == 16) ? 1 : 0;
+    op.param = (int32_t)fp->vals.size();
+    const double *v = p + at;
+    if (kind == TOP_2Q) {
+      for (int r = 0; r < 16; ++r)
+        for (int c = 0; c < 16; ++c)
+          if (r == 0 && v[c] != (c == 0 ? 1.0 : 0.0))
+            return fail(DM_ERR_ARG, "op %d.%d: two-qubit PTM is not trace preserving", opi, i);
+    }
+    if (kind == TOP_DENSE) {
+      const int rows = 1 << (2 * kq);
+      for (int c = 0; c < rows; ++c)
+        for (int r = 0; r < rows; ++r) fp->vals.push_back(v[(size_t)r * rows + c]);  // transposed
+    } else {
+      fp->vals.insert(fp->vals.end(), v, v + nv);
+    }
+    at += nv;
+    fp->ops.push_back(op);
+  }
+  return DM_OK;
+}
+
+template <typename T>
+int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T *d_vals) {
+  ProfScope ps(h, 9);
+  const int grid = (int)std::min<uint64_t>(fp.geo.tiles, (uint64_t)h->cu_count * 8);
+  const int nops = (int)fp.ops.size();
+  if (fp.m == 6)
+    hipLaunchKernelGGL((k_tile_prog<T, 6>), dim3(grid), dim3(kBlock), 0, h->stream, static_cast<T *>(h->d_a),
+                       fp.geo, d_ops, nops, d_vals);
+  else
+    hipLaunchKernelGGL((k_tile_prog<T, 5>), dim3(grid), dim3(kBlock), 0, h->stream, static_cast<T *>(h->d_a),
+                       fp.geo, d_ops, nops, d_vals);
+  return last_launch("k_tile_prog");
+}
+
 template <typename T>
 int run_program(dm_state *h, const dm_op *ops, int nops, const double *params, int64_t nparams) {
   size_t pool = 0;
+  std::vector<FusedPass> fused;
+  std::vector<int> fused_of(nops, -1);
+  size_t tile_ops = 0;
   for (int i = 0; i < nops; ++i) {
     if (int rc = validate_op(h, ops[i], i, nparams)) return rc;
     if (ops[i].kind == DM_OP_BLOCK) pool += (size_t)1 << (4 * ops[i].k);
+    if (ops[i].kind == DM_OP_FUSED) {
+      if (ops[i].param < 0 || ops[i].param >= nparams) return fail(DM_ERR_ARG, "op %d: bad parameter offset", i);
+      FusedPass fp;
+      if (int rc = parse_fused(h, params + ops[i].param, nparams - ops[i].param, i, &fp)) return rc;
+      fused_of[i] = (int)fused.size();
+      pool += fp.vals.size();
+      tile_ops += fp.ops.size();
+      fused.push_back(std::move(fp));
+    }
   }
   std::vector<size_t> offs(nops, 0);
   std::vector<int> pos(4 * (size_t)nops, 0);
+  std::vector<TileOp> tops;
+  std::vector<size_t> top_off(fused.size(), 0);
   if (pool) {
     std::vector<T> staging(pool);
     size_t off = 0;
     for (int i = 0; i < nops; ++i) {
-      if (ops[i].kind != DM_OP_BLOCK) continue;
-      prep_block_matrix<T>(h->n, ops[i].qubits, ops[i].k, params + ops[i].param, staging.data() + off,
-                           &pos[4 * (size_t)i]);
-      offs[i] = off;
-      off += (size_t)1 << (4 * ops[i].k);
+      if (ops[i].kind == DM_OP_BLOCK) {
+        prep_block_matrix<T>(h->n, ops[i].qubits, ops[i].k, params + ops[i].param, staging.data() + off,
+                             &pos[4 * (size_t)i]);
+        offs[i] = off;
+        off += (size_t)1 << (4 * ops[i].k);
+      } else if (ops[i].kind == DM_OP_FUSED) {
+        FusedPass &fp = fused[fused_of[i]];
+        offs[i] = off;
+        for (double v : fp.vals) staging[off++] = (T)v;
+        top_off[fused_of[i]] = tops.size();
+        tops.insert(tops.end(), fp.ops.begin(), fp.ops.end());
+      }
     }
     if (int rc = ensure(&h->d_mat, &h->mat_cap, pool * sizeof(T))) return rc;
-    HIP_TRY(hipStreamSynchronize(h->stream));  // earlier block kernels may still read d_mat
+    HIP_TRY(hipStreamSynchronize(h->stream));  // earlier kernels may still read d_mat / d_ops
     HIP_TRY(hipMemcpy(h->d_mat, staging.data(), pool * sizeof(T), hipMemcpyHostToDevice));
   }
+  if (!tops.empty()) {
+    if (int rc = ensure(&h->d_ops, &h->ops_cap, tops.size() * sizeof(TileOp))) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));  // earlier tile passes may still read d_ops
+    HIP_TRY(hipMemcpy(h->d_ops, tops.data(), tops.size() * sizeof(TileOp), hipMemcpyHostToDevice));
+  }
+  (void)tile_ops;
   const T *dmat = static_cast<const T *>(h->d_mat);
   for (int i = 0; i < nops; ++i) {
     const dm_op &op = ops[i];
@@ -1285,6 +1616,11 @@ int run_program(dm_state *h, const dm_op *ops, int nops, const double *params, i
       case DM_OP_CX_SCALED: rc = launch_cx<T>(h, op.qubits[0], op.qubits[1], pv); break;
       case DM_OP_2Q: rc = launch_2q<T>(h, op.qubits[0], op.qubits[1], pv); break;
       case DM_OP_BLOCK: rc = launch_block_dev<T>(h, op.k, &pos[4 * (size_t)i], dmat + offs[i]); break;
+      case DM_OP_FUSED: {
+        const int f = fused_of[i];
+        rc = launch_fused<T>(h, fused[f], static_cast<const TileOp *>(h->d_ops) + top_off[f], dmat + offs[i]);
+        break;
+      }
       case DM_OP_MEMORY_NOISE:
         if (!(pv[0] == 1.0 && pv[2] == 1.0)) rc = launch_memory_noise<T>(h, pv[0], pv[1], pv[2]);
         break;
@@ -1303,6 +1639,13 @@ int run_program(dm_state *h, const dm_op *ops, int nops, const double *params, i
 }
 
 }  // namespace
+
+namespace dmsim_internal {
+int set_error(int code, const char *msg) {
+  snprintf(g_err, sizeof(g_err), "%s", msg);
+  return code;
+}
+}  // namespace dmsim_internal
 
 // ===========================================================================
 // C ABI
@@ -1367,6 +1710,7 @@ int dm_destroy(dm_state *h) {
   if (h->d_a) hipFree(h->d_a);
   if (h->d_red) hipFree(h->d_red);
   if (h->d_mat) hipFree(h->d_mat);
+  if (h->d_ops) hipFree(h->d_ops);
   if (h->d_aux) hipFree(h->d_aux);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);

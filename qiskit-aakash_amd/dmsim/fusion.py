@@ -4,9 +4,9 @@ A noisy gate is a PTM on one or two qubits (the gate followed by its
 channels).  A *block* is a set of gates on at most k qubits that can be
 applied together: its gates keep their order on every qubit, and every gate
 left out of it that comes earlier in the program acts on other qubits only
-(see ``_grow``), so running the block first commutes past them.  A block of one gate runs on the plain kernels;
-blocks on 2 qubits run the dense 16x16 kernel; blocks on 3 or 4 qubits run the
-MFMA block kernel.  ``plan`` chooses the blocks:
+(see ``_grow``), so running the block first commutes past them.  A block of one gate runs on the plain kernels; a
+larger block is one tile pass (DM_OP_FUSED): the gates applied in sequence to
+LDS-staged tiles, or the block's dense PTM on the matrix cores.  ``plan`` chooses the blocks:
 
 * ``greedy``  -- grow each block up to k_max qubits (the classic rule);
 * ``learned`` -- at every step try k = 1..4 and take the block with the most
@@ -157,9 +157,22 @@ def plan(gates, n, kmax=4, method='greedy', cost=None):
     return blocks
 
 
-def add_block(prog, gates, qs, mem):
-    """Append one block to a Program: plain kernels for a single gate, the dense
-    16x16 kernel for a two-qubit block, the MFMA kernel for 3-4 qubits."""
+def _subop(g):
+    if g.kind == 'u3':
+        return (native.SUB_1Q, list(g.qubits), ptm.rows3(g.R))
+    if g.kind == 'cx':
+        return (native.SUB_CX, list(g.qubits), g.R)
+    return (native.SUB_2Q, list(g.qubits), g.R)
+
+
+def add_block(prog, gates, qs, mem, mode='seq', tile=6):
+    """Append one block to a Program.
+
+    A single gate runs on its plain kernel.  Otherwise, with n >= 6:
+      mode 'seq'   -- one tile pass applying the gates one after another in LDS;
+      mode 'dense' -- one tile pass applying the block PTM on the matrix cores
+                      (3-4 qubits; 1-2 qubit blocks use the dense 16x16 kernel).
+    """
     if len(mem) == 1:
         g = gates[mem[0]]
         if g.kind == 'u3':
@@ -170,12 +183,19 @@ def add_block(prog, gates, qs, mem):
             prog.two_qubit(g.qubits[0], g.qubits[1], g.R)
         return
     sub = [gates[i] for i in mem]
+    n = prog.n
     if len(qs) == 1:
         prog.one_qubit(qs[0], block_ptm(sub, qs))
+    elif mode == 'seq' and n >= tile:
+        prog.fused(tile, qs, [_subop(g) for g in sub])
     elif len(qs) == 2:
-        prog.two_qubit(qs[0], qs[1], block_ptm(sub, qs).reshape(16, 16))
+        # block_ptm's pair index is i_qs0 + 4 i_qs1: qs[1] is the leading qubit
+        prog.two_qubit(qs[1], qs[0], block_ptm(sub, qs).reshape(16, 16))
+    elif n >= 6:
+        prog.fused(6, qs, [(native.SUB_DENSE, list(qs), block_ptm(sub, qs))])
     else:
-        prog.block(qs, block_ptm(sub, qs))
+        for i in mem:
+            add_block(prog, gates, list(gates[i].qubits), [i])
 
 
 def plain_program(gates, n):
@@ -186,11 +206,16 @@ def plain_program(gates, n):
     return prog
 
 
-def fused_program(gates, n, kmax=4, method='greedy', cost=None):
+def fused_program(gates, n, kmax=4, method='greedy', cost=None, mode='seq'):
+    """Program of fused passes; with a cost model each block also gets the
+    cheaper of the sequential and dense forms."""
     blocks = plan(gates, n, kmax, method, cost)
     prog = Program(n)
     for qs, mem in blocks:
-        add_block(prog, gates, qs, mem)
+        m = mode
+        if cost is not None and len(mem) > 1:
+            m = cost.best_mode(n, qs, [gates[i] for i in mem])
+        add_block(prog, gates, qs, mem, m)
     return prog, blocks
 
 

@@ -18,6 +18,7 @@ FP64, FP32 = 0, 1
 c_double_p = ctypes.POINTER(ctypes.c_double)
 c_int_p = ctypes.POINTER(ctypes.c_int)
 c_u64_p = ctypes.POINTER(ctypes.c_uint64)
+c_float_p = ctypes.POINTER(ctypes.c_float)
 
 # name -> (restype, argtypes); every entry point of include/dmsim.h
 SIGNATURES = {
@@ -46,6 +47,13 @@ SIGNATURES = {
     'dm_to_dense': (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
     'dm_run_program': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, c_double_p,
                                       ctypes.c_int64]),
+    'dm_mlp_param_count': (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    'dm_mlp_init': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, c_float_p]),
+    'dm_mlp_predict': (ctypes.c_int, [c_float_p, ctypes.c_int, ctypes.c_int, c_float_p, ctypes.c_int, c_float_p]),
+    'dm_mlp_loss_grad': (ctypes.c_int, [c_float_p, ctypes.c_int, ctypes.c_int, c_float_p, c_float_p, ctypes.c_int,
+                                        c_float_p, c_float_p]),
+    'dm_mlp_train': (ctypes.c_int, [c_float_p, ctypes.c_int, ctypes.c_int, c_float_p, c_float_p, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, c_float_p]),
     'dm_synchronize': (ctypes.c_int, [ctypes.c_void_p]),
     'dm_profile': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     'dm_profile_read': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_u64_p, c_double_p]),
@@ -58,10 +66,12 @@ class DmOp(ctypes.Structure):
                 ('param', ctypes.c_int64)]
 
 
-OP_1Q, OP_CX, OP_CX_SCALED, OP_2Q, OP_BLOCK, OP_MEMORY_NOISE, OP_DIAG = 1, 2, 3, 4, 5, 6, 7
+OP_1Q, OP_CX, OP_CX_SCALED, OP_2Q, OP_BLOCK, OP_MEMORY_NOISE, OP_DIAG, OP_FUSED = 1, 2, 3, 4, 5, 6, 7, 8
+# sub-operation kinds of a fused pass
+SUB_1Q, SUB_CX, SUB_2Q, SUB_DENSE = 1, 2, 3, 4
 
 FAMILIES = {'1q': 0, 'cx': 1, '2q': 2, 'memory_noise': 3, 'diag': 4, 'block3': 5, 'reduce': 6, 'init': 7,
-            'block4': 8}
+            'block4': 8, 'fused': 9}
 
 
 class NativeError(RuntimeError):
@@ -251,3 +261,60 @@ class DeviceState:
         check(self.lib.dm_profile_read(self.h, FAMILIES[family], ctypes.byref(n), ctypes.byref(ms)),
               'dm_profile_read')
         return n.value, ms.value
+
+
+# ---------------------------------------------------------------------------
+# cost-model perceptron
+# ---------------------------------------------------------------------------
+
+def _fp(a):
+    return a.ctypes.data_as(c_float_p)
+
+
+def _f32(x, size=None):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    if size is not None and a.size != size:
+        raise ValueError('expected %d values, got %d' % (size, a.size))
+    return a
+
+
+def mlp_param_count(n_features, hidden):
+    c = load().dm_mlp_param_count(int(n_features), int(hidden))
+    check(c if c < 0 else 0, 'dm_mlp_param_count')
+    return c
+
+
+def mlp_init(n_features, hidden, seed):
+    p = np.zeros(mlp_param_count(n_features, hidden), dtype=np.float32)
+    check(load().dm_mlp_init(int(n_features), int(hidden), int(seed), _fp(p)), 'dm_mlp_init')
+    return p
+
+
+def mlp_predict(params, n_features, hidden, X):
+    X = _f32(X)
+    n = X.shape[0] if X.ndim == 2 else 0
+    out = np.zeros(max(n, 1), dtype=np.float32)
+    check(load().dm_mlp_predict(_fp(_f32(params)), int(n_features), int(hidden), _fp(X), n, _fp(out)),
+          'dm_mlp_predict')
+    return out[:n]
+
+
+def mlp_loss_grad(params, n_features, hidden, X, y):
+    p = _f32(params)
+    X = _f32(X)
+    y = _f32(y)
+    loss = np.zeros(1, dtype=np.float32)
+    grad = np.zeros(p.size, dtype=np.float32)
+    check(load().dm_mlp_loss_grad(_fp(p), int(n_features), int(hidden), _fp(X), _fp(y), y.size, _fp(loss),
+                                  _fp(grad)), 'dm_mlp_loss_grad')
+    return float(loss[0]), grad
+
+
+def mlp_train(params, n_features, hidden, X, y, epochs, batch, lr, seed):
+    p = _f32(params).copy()
+    X = _f32(X)
+    y = _f32(y)
+    losses = np.zeros(max(1, epochs), dtype=np.float32)
+    check(load().dm_mlp_train(_fp(p), int(n_features), int(hidden), _fp(X), _fp(y), y.size, int(epochs),
+                              int(batch), float(lr), int(seed), _fp(losses)), 'dm_mlp_train')
+    return p, losses[:epochs]

@@ -48,6 +48,16 @@ class Program:
     def diag(self, tab_nx4):
         self._add(native.OP_DIAG, [], tab_nx4)
 
+    def fused(self, m, qubits, subops):
+        """One tile pass: ``subops`` is a list of (kind, qubits, values) with
+        kind native.SUB_1Q / SUB_CX / SUB_2Q / SUB_DENSE (see DM_OP_FUSED)."""
+        rec = [float(m), float(len(qubits))] + [float(q) for q in qubits] + [float(len(subops))]
+        for kind, qs, vals in subops:
+            v = [] if vals is None else list(np.asarray(vals, dtype=np.float64).reshape(-1))
+            q4 = list(qs) + [0] * (4 - len(qs))
+            rec += [float(kind), float(len(qs))] + [float(q) for q in q4] + [float(len(v))] + v
+        self._add(native.OP_FUSED, [], rec, k=0)
+
     def kinds(self):
         return [o[0] for o in self._ops]
 

@@ -256,3 +256,42 @@ def test_block_mfma_general_matrix(prec, k, n, qubits):
     t = 1e-15 * 2.0 ** -n * 4 ** k if prec == 'fp64' else 2e-6 * 2.0 ** -n * 4
     assert np.max(np.abs(st.download() - want)) <= t
     st.close()
+
+
+@pytest.mark.parametrize('prec', PRECS)
+@pytest.mark.parametrize('mode', ['seq', 'dense'])
+@pytest.mark.parametrize('n,kmax', [(6, 4), (7, 3), (8, 4), (9, 2)])
+def test_fused_passes_match_plain(prec, mode, n, kmax):
+    """Fused tile passes (sequential and matrix-core forms) against one kernel
+    per gate on a seeded noisy circuit."""
+    from dmsim import fusion, workloads
+    gates = fusion.noisy_gates(workloads.local_u3_cx(n, 120, n), 1e-3)
+    rng = np.random.default_rng(n)
+    a = rand_state(n, rng)
+    plain = fusion.plain_program(gates, n)
+    fused, blocks = fusion.fused_program(gates, n, kmax, 'greedy', None, mode)
+    assert len(fused) < len(plain)
+    s1 = make(n, 'fp64', a)
+    plain.run(s1)
+    want = s1.download()
+    s1.close()
+    s2 = make(n, prec, a)
+    fused.run(s2)
+    got = s2.download()
+    s2.close()
+    t = 1e-14 * 2.0 ** -n * 50 if prec == 'fp64' else 2e-6 * 2.0 ** -n * 50
+    assert np.max(np.abs(got - want)) <= t
+
+
+def test_fused_pass_rejects_bad_records():
+    from dmsim.program import Program
+    st = native.DeviceState(6, 'fp64')
+    p = Program(6)
+    p.fused(6, [0, 1], [(native.SUB_1Q, [9], np.zeros(12))])
+    with pytest.raises(native.NativeError, match='not in the fused tile|bad fused'):
+        p.run(st)
+    p = Program(6)
+    p.fused(4, [0], [])
+    with pytest.raises(native.NativeError, match='tile size'):
+        p.run(st)
+    st.close()
