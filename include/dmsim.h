@@ -137,7 +137,21 @@ int dm_to_dense(dm_state *h, double *out_c128);
  *   DM_OP_2Q            qubits[0..1], 256 values (dense 16x16 PTM)
  *   DM_OP_BLOCK         k = 3 or 4 qubits, 4^(2k) values (block PTM)
  *   DM_OP_MEMORY_NOISE  3 values f, p, g
- *   DM_OP_DIAG          4n values (per-qubit diagonal tables)            */
+ *   DM_OP_DIAG          4n values (per-qubit diagonal tables)
+ *   DM_OP_FUSED         one HBM pass applying several operations to tiles
+ *                       of 4^m coefficients staged in LDS; a variable-length
+ *                       record of values (integers stored exactly):
+ *                         m (5 or 6), nq, q_1 .. q_nq  -- qubits the tile
+ *                           holds; it is completed with the qubits of lowest
+ *                           stride, and must hold qubit n-1
+ *                         nsub, then nsub sub-operations, each
+ *                           kind, kq, q0, q1, q2, q3, nv, v_1 .. v_nv
+ *                         kind 1: 1-qubit PTM rows (kq=1, nv=12)
+ *                         kind 2: CX control q0, target q1 (kq=2, nv=0 or
+ *                                 16 per-pair scales)
+ *                         kind 3: dense 16x16 PTM on (q0,q1) (kq=2, nv=256)
+ *                         kind 4: dense 4^kq x 4^kq PTM on q0..q{kq-1}
+ *                                 (kq=3,4; m=6; nv=4^(2kq); matrix cores)  */
 enum {
   DM_OP_1Q = 1,
   DM_OP_CX = 2,
@@ -145,7 +159,8 @@ enum {
   DM_OP_2Q = 4,
   DM_OP_BLOCK = 5,
   DM_OP_MEMORY_NOISE = 6,
-  DM_OP_DIAG = 7
+  DM_OP_DIAG = 7,
+  DM_OP_FUSED = 8
 };
 
 typedef struct dm_op {
@@ -157,12 +172,44 @@ typedef struct dm_op {
 
 int dm_run_program(dm_state *h, const dm_op *ops, int nops, const double *params, int64_t nparams);
 
+/* Fusion cost model (no reference counterpart): a 3-layer perceptron
+ * x -> relu(x W1 + b1) -> relu(. W2 + b2) -> . W3 + b3, FP32, whose forward,
+ * backward (mean squared error) and Adam kernels run on the GPU.  Parameter
+ * vector: W1 [F x H] row-major, b1 [H], W2 [H x H], b2 [H], W3 [H], b3 [1].
+ * dm_mlp_param_count returns the length (or a negative error code). */
+int dm_mlp_param_count(int n_features, int hidden);
+int dm_mlp_init(int n_features, int hidden, uint64_t seed, float *params);
+int dm_mlp_predict(const float *params, int n_features, int hidden, const float *X, int n, float *out);
+int dm_mlp_loss_grad(const float *params, int n_features, int hidden, const float *X, const float *y, int n,
+                     float *loss, float *grad);
+int dm_mlp_train(float *params, int n_features, int hidden, const float *X, const float *y, int n, int epochs,
+                 int batch, float lr, uint64_t seed, float *loss_per_epoch);
+
+/* Batches of independent circuits (no reference counterpart; the reference
+ * runs experiments one after another, dm_simulator.py:935-936).
+ * Circuit c has n_qubits[c] qubits and the program ops[op_start[c] ..
+ * op_start[c+1]) (dm_op records over the shared params array).  Each circuit
+ * starts in |0...0>, runs its program and ends with the Z-basis ensemble
+ * probabilities of all its qubits with polarisation retention meas_err[c]
+ * (2^n values, most significant bit = qubit 0, circuits concatenated).
+ * Circuits of at most lds_max_qubits (<= 7) qubits run whole inside one
+ * workgroup's LDS (one launch per width, lds_threads threads per workgroup);
+ * the others keep their state in HBM and run on 8 parallel streams.
+ * dm_batch_run with use_graph != 0 captures the whole batch into one hipGraph
+ * on first use and replays it; per-circuit times come from GPU clock stamps. */
+typedef struct dm_batch dm_batch;
+int dm_batch_create(int dtype, int ncirc, const int *n_qubits, const int64_t *op_start, const dm_op *ops,
+                    const double *params, int64_t nparams, const double *meas_err, int lds_max_qubits,
+                    int lds_threads, dm_batch **out);
+int dm_batch_run(dm_batch *b, int use_graph, double *probs_out, double *circuit_us, double *total_ms);
+int dm_batch_destroy(dm_batch *b);
+
 /* Stream control and timing (no reference counterpart). */
 int dm_synchronize(dm_state *h);
 /* Accumulated GPU time per kernel family since the last reset, measured with
  * HIP events around each launch when profiling is on. family: 0 = 1q, 1 = cx,
  * 2 = 2q dense, 3 = memory noise, 4 = diag, 5 = block k=3, 6 = reductions,
- * 7 = init/copies, 8 = block k=4. */
+ * 7 = init/copies, 8 = block k=4, 9 = fused tile passes. */
 int dm_profile(dm_state *h, int enable);
 int dm_profile_read(dm_state *h, int family, uint64_t *launches, double *ms);
 int dm_profile_reset(dm_state *h);

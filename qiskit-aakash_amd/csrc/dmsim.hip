@@ -662,7 +662,13 @@ __global__ __launch_bounds__(kBlock) void k_tile_prog(T *__restrict__ a, TileGeo
   const int tid = threadIdx.x;
   const int w = geo.w;
   const int lomask = (1 << (2 * w)) - 1;
-  for (uint64_t tile = blockIdx.x; tile < geo.tiles; tile += gridDim.x) {
+  // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+  // L2).  Neighbouring tiles share HBM lines (runs of 4^w coefficients are
+  // shorter than a line when w is small), so consecutive tiles go to
+  // workgroups of the same XCD: logical id = (b % 8) * (grid / 8) + b / 8.
+  const uint64_t g8 = gridDim.x / 8;
+  const uint64_t lid = (gridDim.x % 8 == 0) ? (blockIdx.x % 8) * g8 + blockIdx.x / 8 : blockIdx.x;
+  for (uint64_t tile = lid; tile < geo.tiles; tile += gridDim.x) {
     uint64_t base = 0;
     {
       uint64_t r = tile;
@@ -690,6 +696,20 @@ __global__ __launch_bounds__(kBlock) void k_tile_prog(T *__restrict__ a, TileGeo
         for (int x = 0; x < 3; ++x)
 #pragma unroll
           for (int y = 0; y < 4; ++y) r[x][y] = pv[4 * x + y];
+        if (j == 0) {
+          // the four components of a group are adjacent: one vector LDS access
+          for (int g = tid; g < E / 4; g += kBlock) {
+            Pack<T, 4> x = *reinterpret_cast<const Pack<T, 4> *>(&s[4 * g]);
+            Pack<T, 4> y;
+            y.v[0] = x.v[0];
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+              y.v[q + 1] = r[q][0] * x.v[0] + r[q][1] * x.v[1] + r[q][2] * x.v[2] + r[q][3] * x.v[3];
+            *reinterpret_cast<Pack<T, 4> *>(&s[4 * g]) = y;
+          }
+          __syncthreads();
+          continue;
+        }
         for (int g = tid; g < E / 4; g += kBlock) {
           const int b = ins_zero(g, j);
           const T x0 = s[b], x1 = s[b + st], x2 = s[b + 2 * st], x3 = s[b + 3 * st];
@@ -908,6 +928,131 @@ __global__ __launch_bounds__(kBlock) void k_dense_scatter(const double2 *__restr
 
 }  // namespace
 
+namespace {
+
+// ---------------------------------------------------------------------------
+// K8: whole-circuit kernel for small registers (n <= 7).  One workgroup holds
+// one circuit's entire state (4^n coefficients, at most 128 KB in FP64) in
+// LDS, applies its operations (tile ops over all n digits; TOP_ALLQ applies
+// one 1q map to every qubit, the per-level memory noise), then writes the
+// Z-basis ensemble probabilities 2^n values and its start / end GPU clock.
+// ---------------------------------------------------------------------------
+
+enum { TOP_ALLQ = 5 };
+
+struct BatchCirc {
+  int32_t n, nops;
+  int64_t op_off;   // first TileOp
+  int64_t out_off;  // first probability
+  double err;       // measurement polarisation retention
+};
+
+__device__ __forceinline__ void stamp_now(uint64_t *where) { *where = wall_clock64(); }
+
+__global__ void k_stamp(uint64_t *where) { stamp_now(where); }
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_circuit_lds(const BatchCirc *__restrict__ circs, const TileOp *__restrict__ allops,
+                                                      const T *__restrict__ prm, double *__restrict__ probs,
+                                                      uint64_t *__restrict__ stamps) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *s = reinterpret_cast<T *>(smem_raw);
+  const BatchCirc c = circs[blockIdx.x];
+  const int n = c.n;
+  const int E = 1 << (2 * n);
+  const int tid = threadIdx.x, nt = blockDim.x;
+  if (tid == 0) stamp_now(&stamps[2 * blockIdx.x]);
+  const T amp = (T)ldexp(1.0, -n);
+  for (int i = tid; i < E; i += nt) {
+    // |0...0>: 2^-n on indices whose digits are all I or Z
+    const uint32_t lo = (uint32_t)i & 0x55555555u, hi = ((uint32_t)i >> 1) & 0x55555555u;
+    s[i] = (lo ^ hi) ? T(0) : amp;
+  }
+  __syncthreads();
+  const TileOp *ops = allops + c.op_off;
+  for (int oi = 0; oi < c.nops; ++oi) {
+    const TileOp op = ops[oi];
+    const T *pv = prm + op.param;
+    if (op.kind == TOP_1Q || op.kind == TOP_ALLQ) {
+      const int j0 = op.kind == TOP_1Q ? op.d[0] : 0;
+      const int j1 = op.kind == TOP_1Q ? op.d[0] + 1 : n;
+      for (int j = j0; j < j1; ++j) {
+        const int st = 1 << (2 * j);
+        for (int g = tid; g < E / 4; g += nt) {
+          const int b = ins_zero(g, j);
+          const T x0 = s[b], x1 = s[b + st], x2 = s[b + 2 * st], x3 = s[b + 3 * st];
+          s[b + st] = pv[0] * x0 + pv[1] * x1 + pv[2] * x2 + pv[3] * x3;
+          s[b + 2 * st] = pv[4] * x0 + pv[5] * x1 + pv[6] * x2 + pv[7] * x3;
+          s[b + 3 * st] = pv[8] * x0 + pv[9] * x1 + pv[10] * x2 + pv[11] * x3;
+        }
+        __syncthreads();
+      }
+      continue;
+    }
+    if (op.kind == TOP_CX || op.kind == TOP_2Q) {
+      const int jc = op.d[0], jt = op.d[1];
+      const int jl = min(jc, jt), jh = max(jc, jt);
+      const int sc = 1 << (2 * jc), stt = 1 << (2 * jt);
+      for (int g = tid; g < E / 16; g += nt) {
+        const int b = ins_zero(ins_zero(g, jl), jh);
+        T v[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int t = 0; t < 4; ++t) v[a][t] = s[b + a * sc + t * stt];
+        if (op.kind == TOP_CX) {
+          Scale16<T> sv;
+#pragma unroll
+          for (int q = 0; q < 16; ++q) sv.s[q] = op.k ? pv[q] : T(1);
+          cx_map<T, T, true, true>(v, sv);
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+              if (a | t) s[b + a * sc + t * stt] = v[a][t];
+        } else {
+#pragma unroll
+          for (int r = 1; r < 16; ++r) {
+            T acc = T(0);
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc += pv[16 * r + q] * v[q >> 2][q & 3];
+            s[b + (r >> 2) * sc + (r & 3) * stt] = acc;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // Z ensemble: gather the {I,Z}^n sub-lattice into the free LDS after the
+  // state (2^n values), contract one qubit at a time in place (FP64).
+  double *v = reinterpret_cast<double *>(smem_raw + (size_t)E * sizeof(T));
+  const int M = 1 << n;
+  for (int x = tid; x < M; x += nt) {
+    int idx = 0;
+    for (int q = 0; q < n; ++q)  // bit q of x -> qubit n-1-q -> position q
+      if ((x >> q) & 1) idx |= 3 << (2 * q);
+    v[x] = (double)s[idx];
+  }
+  __syncthreads();
+  for (int q = 0; q < n; ++q) {  // butterfly on bit q: (u, w) -> (u + e w, u - e w)
+    for (int x = tid; x < M / 2; x += nt) {
+      const int lo = x & ((1 << q) - 1);
+      const int b = ((x >> q) << (q + 1)) | lo;
+      const double u = v[b], w = v[b | (1 << q)];
+      v[b] = u + c.err * w;
+      v[b | (1 << q)] = u - c.err * w;
+    }
+    __syncthreads();
+  }
+  // outcome b: bit of qubit k is bit (n-1-k) of b (qubit 0 most significant);
+  // in v, qubit k is bit n-1-k as well, so the order already matches
+  for (int x = tid; x < M; x += nt) probs[c.out_off + x] = v[x];
+  __syncthreads();
+  if (tid == 0) stamp_now(&stamps[2 * blockIdx.x + 1]);
+}
+
+}  // namespace
+
 // ===========================================================================
 // Host side
 // ===========================================================================
@@ -930,6 +1075,7 @@ struct dm_state {
   int cu_count = 256;
   void *d_a = nullptr;
   hipStream_t stream = nullptr;
+  bool owns_stream = true;
   // scratch
   double *d_red = nullptr;      // reduction partials + results
   size_t red_cap = 0;           // doubles
@@ -1544,7 +1690,8 @@ int parse_fused(const dm_state *h, const double *p, int64_t avail, int opi, Fuse
 template <typename T>
 int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T *d_vals) {
   ProfScope ps(h, 9);
-  const int grid = (int)std::min<uint64_t>(fp.geo.tiles, (uint64_t)h->cu_count * 8);
+  int grid = (int)std::min<uint64_t>(fp.geo.tiles, (uint64_t)h->cu_count * 8);
+  if (grid >= 8) grid -= grid % 8;  // whole XCD rounds (see k_tile_prog)
   const int nops = (int)fp.ops.size();
   if (fp.m == 6)
     hipLaunchKernelGGL((k_tile_prog<T, 6>), dim3(grid), dim3(kBlock), 0, h->stream, static_cast<T *>(h->d_a),
@@ -1555,12 +1702,24 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
   return last_launch("k_tile_prog");
 }
 
-template <typename T>
-int run_program(dm_state *h, const dm_op *ops, int nops, const double *params, int64_t nparams) {
-  size_t pool = 0;
+// A program split into an upload phase (prepare) and a launch phase that only
+// enqueues kernels on the handle's stream (so it can be captured in a graph).
+struct Prepared {
+  std::vector<dm_op> ops;
+  std::vector<double> params;
   std::vector<FusedPass> fused;
-  std::vector<int> fused_of(nops, -1);
-  size_t tile_ops = 0;
+  std::vector<int> fused_of;
+  std::vector<size_t> offs, top_off;
+  std::vector<int> pos;
+};
+
+template <typename T>
+int prepare_program(dm_state *h, const dm_op *ops, int nops, const double *params, int64_t nparams, Prepared *pp) {
+  pp->ops.assign(ops, ops + nops);
+  pp->params.assign(params, params + (nparams > 0 ? nparams : 0));
+  pp->fused.clear();
+  pp->fused_of.assign(nops, -1);
+  size_t pool = 0;
   for (int i = 0; i < nops; ++i) {
     if (int rc = validate_op(h, ops[i], i, nparams)) return rc;
     if (ops[i].kind == DM_OP_BLOCK) pool += (size_t)1 << (4 * ops[i].k);
@@ -1568,30 +1727,29 @@ int run_program(dm_state *h, const dm_op *ops, int nops, const double *params, i
       if (ops[i].param < 0 || ops[i].param >= nparams) return fail(DM_ERR_ARG, "op %d: bad parameter offset", i);
       FusedPass fp;
       if (int rc = parse_fused(h, params + ops[i].param, nparams - ops[i].param, i, &fp)) return rc;
-      fused_of[i] = (int)fused.size();
+      pp->fused_of[i] = (int)pp->fused.size();
       pool += fp.vals.size();
-      tile_ops += fp.ops.size();
-      fused.push_back(std::move(fp));
+      pp->fused.push_back(std::move(fp));
     }
   }
-  std::vector<size_t> offs(nops, 0);
-  std::vector<int> pos(4 * (size_t)nops, 0);
+  pp->offs.assign(nops, 0);
+  pp->pos.assign(4 * (size_t)nops, 0);
+  pp->top_off.assign(pp->fused.size(), 0);
   std::vector<TileOp> tops;
-  std::vector<size_t> top_off(fused.size(), 0);
   if (pool) {
     std::vector<T> staging(pool);
     size_t off = 0;
     for (int i = 0; i < nops; ++i) {
       if (ops[i].kind == DM_OP_BLOCK) {
         prep_block_matrix<T>(h->n, ops[i].qubits, ops[i].k, params + ops[i].param, staging.data() + off,
-                             &pos[4 * (size_t)i]);
-        offs[i] = off;
+                             &pp->pos[4 * (size_t)i]);
+        pp->offs[i] = off;
         off += (size_t)1 << (4 * ops[i].k);
       } else if (ops[i].kind == DM_OP_FUSED) {
-        FusedPass &fp = fused[fused_of[i]];
-        offs[i] = off;
+        FusedPass &fp = pp->fused[pp->fused_of[i]];
+        pp->offs[i] = off;
         for (double v : fp.vals) staging[off++] = (T)v;
-        top_off[fused_of[i]] = tops.size();
+        pp->top_off[pp->fused_of[i]] = tops.size();
         tops.insert(tops.end(), fp.ops.begin(), fp.ops.end());
       }
     }
@@ -1604,21 +1762,27 @@ int run_program(dm_state *h, const dm_op *ops, int nops, const double *params, i
     HIP_TRY(hipStreamSynchronize(h->stream));  // earlier tile passes may still read d_ops
     HIP_TRY(hipMemcpy(h->d_ops, tops.data(), tops.size() * sizeof(TileOp), hipMemcpyHostToDevice));
   }
-  (void)tile_ops;
+  return DM_OK;
+}
+
+template <typename T>
+int launch_program(dm_state *h, const Prepared &pp) {
   const T *dmat = static_cast<const T *>(h->d_mat);
-  for (int i = 0; i < nops; ++i) {
-    const dm_op &op = ops[i];
-    const double *pv = params ? params + op.param : nullptr;
+  const double *params = pp.params.data();
+  for (size_t i = 0; i < pp.ops.size(); ++i) {
+    const dm_op &op = pp.ops[i];
+    const double *pv = pp.params.empty() ? nullptr : params + op.param;
     int rc = DM_OK;
     switch (op.kind) {
       case DM_OP_1Q: rc = launch_1q<T>(h, op.qubits[0], pv); break;
       case DM_OP_CX: rc = launch_cx<T>(h, op.qubits[0], op.qubits[1], nullptr); break;
       case DM_OP_CX_SCALED: rc = launch_cx<T>(h, op.qubits[0], op.qubits[1], pv); break;
       case DM_OP_2Q: rc = launch_2q<T>(h, op.qubits[0], op.qubits[1], pv); break;
-      case DM_OP_BLOCK: rc = launch_block_dev<T>(h, op.k, &pos[4 * (size_t)i], dmat + offs[i]); break;
+      case DM_OP_BLOCK: rc = launch_block_dev<T>(h, op.k, &pp.pos[4 * i], dmat + pp.offs[i]); break;
       case DM_OP_FUSED: {
-        const int f = fused_of[i];
-        rc = launch_fused<T>(h, fused[f], static_cast<const TileOp *>(h->d_ops) + top_off[f], dmat + offs[i]);
+        const int f = pp.fused_of[i];
+        rc = launch_fused<T>(h, pp.fused[f], static_cast<const TileOp *>(h->d_ops) + pp.top_off[f],
+                             dmat + pp.offs[i]);
         break;
       }
       case DM_OP_MEMORY_NOISE:
@@ -1631,11 +1795,198 @@ int run_program(dm_state *h, const dm_op *ops, int nops, const double *params, i
         rc = launch_diag_tab<T>(h, tab, false, 1.0);
         break;
       }
-      default: rc = fail(DM_ERR_ARG, "op %d: unknown kind", i);
+      default: rc = fail(DM_ERR_ARG, "op %zu: unknown kind", i);
     }
     if (rc) return rc;
   }
   return DM_OK;
+}
+
+template <typename T>
+int run_program(dm_state *h, const dm_op *ops, int nops, const double *params, int64_t nparams) {
+  Prepared pp;
+  if (int rc = prepare_program<T>(h, ops, nops, params, nparams, &pp)) return rc;
+  return launch_program<T>(h, pp);
+}
+
+// Capturable Z-ensemble probabilities of one HBM-resident state into a device
+// buffer (gather + per-qubit contraction).
+template <typename T>
+int enqueue_probs(dm_state *h, double err, double *dst, double *scratch) {
+  GatherSpec gs;
+  memset(&gs, 0, sizeof(gs));
+  gs.m = h->n;
+  gs.base_log = 1;
+  gs.comp = 3;
+  for (int t = 0; t < h->n; ++t) gs.stride[t] = pow4(h->n - 1 - t);
+  const uint64_t count = 1ull << h->n;
+  hipLaunchKernelGGL((k_gather<T>), dim3(grid_for(h, count)), dim3(kBlock), 0, h->stream,
+                     static_cast<const T *>(h->d_a), count, gs, scratch);
+  double *src = scratch, *buf = scratch + count;
+  for (int t = 0; t < h->n; ++t) {
+    const uint64_t pre = 1ull << t, post = 1ull << (h->n - 1 - t);
+    double *out = (t == h->n - 1) ? dst : buf;
+    hipLaunchKernelGGL(k_contract, dim3(grid_for(h, pre * 2 * post)), dim3(kBlock), 0, h->stream, src, out, pre, 2,
+                       post, 1.0, err, 0.0, 0.0, 1.0, -err, 0.0, 0.0);
+    std::swap(src, buf);
+    if (t == h->n - 1) break;
+    src = out;
+    buf = (out == scratch + count) ? scratch : scratch + count;
+  }
+  return last_launch("enqueue_probs");
+}
+
+
+}  // namespace
+
+// A batch of independent circuits run together (see dm_batch_create).
+struct dm_batch {
+  int dtype = DM_FP64;
+  int ncirc = 0;
+  std::vector<int> nq;
+  std::vector<int64_t> prob_off;
+  int64_t nprobs = 0;
+  // circuits held entirely in LDS, grouped by width
+  std::vector<int> small;
+  std::vector<int> small_width_start;  // per width 1..7: first index into small-ordered arrays
+  void *d_circs = nullptr, *d_tops = nullptr, *d_vals = nullptr;
+  int lds_threads = 256;
+  // circuits whose state lives in HBM
+  std::vector<int> large;
+  std::vector<dm_state *> states;
+  std::vector<Prepared> progs;
+  std::vector<double> err;
+  std::vector<double *> scratch;
+  hipStream_t master = nullptr;
+  std::vector<hipStream_t> lanes;
+  hipEvent_t fork = nullptr, ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> joins;
+  double *d_probs = nullptr;
+  uint64_t *d_stamps = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double clock_khz = 1.0e5;
+};
+
+namespace {
+
+// Tile ops of a small circuit: every position is a tile digit (digit j = position j).
+int small_circuit_ops(int n, const dm_op *ops, int nops, const double *params, int64_t nparams, int ci,
+                      std::vector<TileOp> *tops, std::vector<double> *vals) {
+  auto digit = [&](int q) { return n - 1 - q; };
+  auto push = [&](int kind, int k, const int *qs, const double *v, int nv) {
+    TileOp op;
+    memset(&op, 0, sizeof(op));
+    op.kind = kind;
+    op.k = k;
+    for (int x = 0; x < 4; ++x) op.d[x] = qs ? digit(qs[x] < 0 ? 0 : qs[x]) : 0;
+    op.param = (int32_t)vals->size();
+    vals->insert(vals->end(), v, v + nv);
+    tops->push_back(op);
+  };
+  dm_state probe;
+  probe.n = n;
+  for (int i = 0; i < nops; ++i) {
+    const dm_op &op = ops[i];
+    if (int rc = validate_op(&probe, op, i, nparams)) return rc;
+    const double *pv = params + op.param;
+    int qs[4] = {op.qubits[0], op.qubits[1], op.qubits[2], op.qubits[3]};
+    switch (op.kind) {
+      case DM_OP_1Q: push(TOP_1Q, 1, qs, pv, 12); break;
+      case DM_OP_CX: push(TOP_CX, 0, qs, pv, 0); break;
+      case DM_OP_CX_SCALED: push(TOP_CX, 1, qs, pv, 16); break;
+      case DM_OP_2Q: {
+        for (int c = 0; c < 16; ++c)
+          if (pv[c] != (c == 0 ? 1.0 : 0.0))
+            return fail(DM_ERR_ARG, "circuit %d op %d: two-qubit PTM is not trace preserving", ci, i);
+        push(TOP_2Q, 2, qs, pv, 256);
+        break;
+      }
+      case DM_OP_MEMORY_NOISE: {
+        const double f = pv[0], p = pv[1], g = pv[2];
+        if (f == 1.0 && g == 1.0) break;
+        const double od = sqrt(g) * f, dd = (1 - g) * (2 * p - 1);
+        const double R[12] = {0, od, 0, 0, 0, 0, od, 0, dd, 0, 0, g};
+        push(TOP_ALLQ, n, nullptr, R, 12);
+        break;
+      }
+      case DM_OP_FUSED: {
+        // the whole register is the tile: inline the sub-operations
+        // (record: m, nq, q..., nsub, then (kind, kq, q0..q3, nv, v...) each)
+        int64_t at = op.param;
+        const double *p = params;
+        if (at + 2 > nparams) return fail(DM_ERR_ARG, "circuit %d op %d: truncated fused record", ci, i);
+        const int nq = (int)p[at + 1];
+        at += 2 + nq;
+        if (nq < 0 || at + 1 > nparams) return fail(DM_ERR_ARG, "circuit %d op %d: truncated fused record", ci, i);
+        const int nsub = (int)p[at++];
+        for (int j = 0; j < nsub; ++j) {
+          if (at + 7 > nparams) return fail(DM_ERR_ARG, "circuit %d op %d: truncated fused record", ci, i);
+          const int kind = (int)p[at], kq = (int)p[at + 1];
+          int sq[4] = {(int)p[at + 2], (int)p[at + 3], (int)p[at + 4], (int)p[at + 5]};
+          const int nv = (int)p[at + 6];
+          at += 7;
+          if (nv < 0 || at + nv > nparams) return fail(DM_ERR_ARG, "circuit %d op %d: bad fused values", ci, i);
+          for (int x = 0; x < kq; ++x)
+            if (sq[x] < 0 || sq[x] >= n) return fail(DM_ERR_ARG, "circuit %d op %d: bad qubit", ci, i);
+          if (kind == TOP_1Q && kq == 1 && nv == 12) push(TOP_1Q, 1, sq, p + at, 12);
+          else if (kind == TOP_CX && kq == 2 && (nv == 0 || nv == 16)) push(TOP_CX, nv ? 1 : 0, sq, p + at, nv);
+          else if (kind == TOP_2Q && kq == 2 && nv == 256) push(TOP_2Q, 2, sq, p + at, 256);
+          else return fail(DM_ERR_ARG, "circuit %d op %d: sub-op kind %d not supported in LDS circuits", ci, i, kind);
+          at += nv;
+        }
+        break;
+      }
+      default:
+        return fail(DM_ERR_ARG, "circuit %d op %d: kind %d not supported in LDS circuits", ci, i, op.kind);
+    }
+  }
+  return DM_OK;
+}
+
+size_t lds_bytes(int n, int dtype) {
+  return ((size_t)1 << (2 * n)) * (dtype == DM_FP64 ? 8 : 4) + ((size_t)1 << n) * 8;
+}
+
+template <typename T>
+int enqueue_batch(dm_batch *b) {
+  HIP_TRY(hipEventRecord(b->fork, b->master));
+  for (auto ln : b->lanes) HIP_TRY(hipStreamWaitEvent(ln, b->fork, 0));
+  // LDS circuits: one launch per width, spread over the lanes
+  for (int w = 1; w <= 7; ++w) {
+    const int lo = b->small_width_start[w], hi = b->small_width_start[w + 1];
+    if (hi <= lo) continue;
+    hipStream_t st = b->lanes[w % b->lanes.size()];
+    const size_t bytes = lds_bytes(w, b->dtype);
+    hipLaunchKernelGGL((k_circuit_lds<T>), dim3(hi - lo), dim3(b->lds_threads), bytes, st,
+                       static_cast<const BatchCirc *>(b->d_circs) + lo, static_cast<const TileOp *>(b->d_tops),
+                       static_cast<const T *>(b->d_vals), b->d_probs, b->d_stamps + 2 * (size_t)lo);
+    if (int rc = last_launch("k_circuit_lds")) return rc;
+  }
+  // HBM circuits: init, program, probabilities, bracketed by clock stamps
+  const size_t ns = b->small.size();
+  for (size_t j = 0; j < b->large.size(); ++j) {
+    dm_state *h = b->states[j];
+    const int c = b->large[j];
+    uint64_t *st = b->d_stamps + 2 * (ns + j);
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, h->stream, st);
+    double tab[4 * kMaxQubits];
+    for (int p = 0; p < h->n; ++p) {
+      tab[4 * p] = 1.0;
+      tab[4 * p + 1] = 0.0;
+      tab[4 * p + 2] = 0.0;
+      tab[4 * p + 3] = 1.0;
+    }
+    if (int rc = launch_diag_tab<T>(h, tab, true, ldexp(1.0, -h->n))) return rc;
+    if (int rc = launch_program<T>(h, b->progs[j])) return rc;
+    if (int rc = enqueue_probs<T>(h, b->err[c], b->d_probs + b->prob_off[c], b->scratch[j])) return rc;
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, h->stream, st + 1);
+  }
+  for (size_t l = 0; l < b->lanes.size(); ++l) {
+    HIP_TRY(hipEventRecord(b->joins[l], b->lanes[l]));
+    HIP_TRY(hipStreamWaitEvent(b->master, b->joins[l], 0));
+  }
+  return last_launch("enqueue_batch");
 }
 
 }  // namespace
@@ -1650,6 +2001,113 @@ int set_error(int code, const char *msg) {
 // ===========================================================================
 // C ABI
 // ===========================================================================
+
+extern "C" int dm_batch_destroy(dm_batch *b);
+
+static int batch_state(int n, int dtype, hipStream_t stream, int cu, dm_state **out) {
+  dm_state *h = new dm_state();
+  h->n = n;
+  h->dtype = dtype;
+  h->N = pow4(n);
+  h->cu_count = cu;
+  h->stream = stream;
+  h->owns_stream = false;
+  hipGetDevice(&h->device);
+  hipError_t e = hipMalloc(&h->d_a, h->N * (dtype == DM_FP64 ? 8 : 4));
+  if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+  if (e != hipSuccess) {
+    const int rc = fail(DM_ERR_HIP, "batch state (%d qubits): %s", n, hipGetErrorString(e));
+    dm_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return DM_OK;
+}
+
+template <typename T>
+static int batch_create_t(dm_batch *b, const int64_t *op_start, const dm_op *ops, const double *params,
+                          int64_t nparams, int lds_max_qubits) {
+  int cu = 256;
+  {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      cu = prop.multiProcessorCount;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+      b->clock_khz = khz;
+  }
+  HIP_TRY(hipStreamCreateWithFlags(&b->master, hipStreamNonBlocking));
+  const int nlanes = 8;
+  for (int l = 0; l < nlanes; ++l) {
+    hipStream_t st;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    b->lanes.push_back(st);
+    hipEvent_t e;
+    HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b->joins.push_back(e);
+  }
+  HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  HIP_TRY(hipEventCreate(&b->ev0));
+  HIP_TRY(hipEventCreate(&b->ev1));
+  // LDS circuits grouped by width (probabilities stay in circuit order)
+  std::vector<BatchCirc> circs;
+  std::vector<TileOp> tops;
+  std::vector<double> vals;
+  b->small_width_start.assign(9, 0);
+  for (int w = 1; w <= 7; ++w) {
+    b->small_width_start[w] = (int)b->small.size();
+    if (w > lds_max_qubits) continue;
+    for (int c = 0; c < b->ncirc; ++c) {
+      if (b->nq[c] != w) continue;
+      BatchCirc bc;
+      bc.n = w;
+      bc.op_off = (int64_t)tops.size();
+      if (int rc = small_circuit_ops(w, ops + op_start[c], (int)(op_start[c + 1] - op_start[c]), params, nparams, c,
+                                     &tops, &vals))
+        return rc;
+      bc.nops = (int32_t)(tops.size() - bc.op_off);
+      bc.out_off = b->prob_off[c];
+      bc.err = b->err[c];
+      circs.push_back(bc);
+      b->small.push_back(c);
+    }
+  }
+  b->small_width_start[8] = (int)b->small.size();
+  if (!circs.empty()) {
+    std::vector<T> tv(vals.begin(), vals.end());
+    HIP_TRY(hipMalloc(&b->d_circs, circs.size() * sizeof(BatchCirc)));
+    HIP_TRY(hipMemcpy(b->d_circs, circs.data(), circs.size() * sizeof(BatchCirc), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&b->d_tops, std::max<size_t>(1, tops.size()) * sizeof(TileOp)));
+    if (!tops.empty()) HIP_TRY(hipMemcpy(b->d_tops, tops.data(), tops.size() * sizeof(TileOp), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&b->d_vals, std::max<size_t>(1, tv.size()) * sizeof(T)));
+    if (!tv.empty()) HIP_TRY(hipMemcpy(b->d_vals, tv.data(), tv.size() * sizeof(T), hipMemcpyHostToDevice));
+    size_t need = 0;
+    for (auto &bc : circs) need = std::max(need, lds_bytes(bc.n, b->dtype));
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_circuit_lds<T>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+  }
+  // HBM circuits: own state, prepared program, probability scratch
+  for (int c = 0; c < b->ncirc; ++c) {
+    if (b->nq[c] <= lds_max_qubits) continue;
+    dm_state *h = nullptr;
+    if (int rc = batch_state(b->nq[c], b->dtype, b->lanes[b->large.size() % b->lanes.size()], cu, &h)) return rc;
+    b->states.push_back(h);
+    b->large.push_back(c);
+    Prepared pp;
+    if (int rc = prepare_program<T>(h, ops + op_start[c], (int)(op_start[c + 1] - op_start[c]), params, nparams, &pp))
+      return rc;
+    b->progs.push_back(std::move(pp));
+    double *sc = nullptr;
+    HIP_TRY(hipMalloc(&sc, 2 * ((size_t)1 << h->n) * sizeof(double)));
+    b->scratch.push_back(sc);
+  }
+  HIP_TRY(hipMalloc(&b->d_probs, std::max<int64_t>(1, b->nprobs) * sizeof(double)));
+  HIP_TRY(hipMalloc(&b->d_stamps, 2 * (size_t)b->ncirc * sizeof(uint64_t)));
+  HIP_TRY(hipDeviceSynchronize());
+  return DM_OK;
+}
 
 #define DISPATCH(h, fn, ...) ((h)->dtype == DM_FP64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
 
@@ -1719,7 +2177,7 @@ int dm_destroy(dm_state *h) {
     hipEventDestroy(pp.b);
   }
   for (auto e : h->spare_events) hipEventDestroy(e);
-  if (h->stream) hipStreamDestroy(h->stream);
+  if (h->stream && h->owns_stream) hipStreamDestroy(h->stream);
   delete h;
   return DM_OK;
 }
@@ -1934,6 +2392,104 @@ int dm_run_program(dm_state *h, const dm_op *ops, int nops, const double *params
   if (nops < 0 || (nops > 0 && !ops)) return fail(DM_ERR_ARG, "bad op list");
   if (nparams < 0 || (nparams > 0 && !params)) return fail(DM_ERR_ARG, "bad parameter array");
   return DISPATCH(h, run_program, h, ops, nops, params, nparams);
+}
+
+int dm_batch_destroy(dm_batch *b) {
+  if (!b) return DM_OK;
+  if (b->master) hipStreamSynchronize(b->master);
+  for (auto ln : b->lanes) hipStreamSynchronize(ln);
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  for (auto h : b->states) dm_destroy(h);
+  for (auto p : b->scratch) hipFree(p);
+  for (void *p : {b->d_circs, b->d_tops, b->d_vals, (void *)b->d_probs, (void *)b->d_stamps})
+    if (p) hipFree(p);
+  for (auto e : b->joins) hipEventDestroy(e);
+  for (hipEvent_t e : {b->fork, b->ev0, b->ev1})
+    if (e) hipEventDestroy(e);
+  for (auto ln : b->lanes) hipStreamDestroy(ln);
+  if (b->master) hipStreamDestroy(b->master);
+  delete b;
+  return DM_OK;
+}
+
+int dm_batch_create(int dtype, int ncirc, const int *n_qubits, const int64_t *op_start, const dm_op *ops,
+                    const double *params, int64_t nparams, const double *meas_err, int lds_max_qubits,
+                    int lds_threads, dm_batch **out) {
+  if (!out) return fail(DM_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (dtype != DM_FP64 && dtype != DM_FP32) return fail(DM_ERR_ARG, "unknown dtype %d", dtype);
+  if (ncirc < 1 || !n_qubits || !op_start || !meas_err) return fail(DM_ERR_ARG, "bad circuit list");
+  if (lds_threads != 256 && lds_threads != 512 && lds_threads != 1024)
+    return fail(DM_ERR_ARG, "lds_threads must be 256, 512 or 1024");
+  if (lds_max_qubits > 7) return fail(DM_ERR_ARG, "LDS circuits hold at most 7 qubits");
+  if (dtype == DM_FP64 && lds_max_qubits > 7) return fail(DM_ERR_ARG, "bad LDS width");
+  dm_batch *b = new dm_batch();
+  b->dtype = dtype;
+  b->ncirc = ncirc;
+  b->lds_threads = lds_threads;
+  for (int c = 0; c < ncirc; ++c) {
+    if (n_qubits[c] < 1 || n_qubits[c] > 16) {
+      dm_batch_destroy(b);
+      return fail(DM_ERR_ARG, "circuit %d: %d qubits outside [1, 16]", c, n_qubits[c]);
+    }
+    if (op_start[c + 1] < op_start[c]) {
+      dm_batch_destroy(b);
+      return fail(DM_ERR_ARG, "circuit %d: bad op range", c);
+    }
+    b->nq.push_back(n_qubits[c]);
+    b->prob_off.push_back(b->nprobs);
+    b->nprobs += (int64_t)1 << n_qubits[c];
+    b->err.push_back(meas_err[c]);
+  }
+  const int rc = dtype == DM_FP64 ? batch_create_t<double>(b, op_start, ops, params, nparams, lds_max_qubits)
+                                  : batch_create_t<float>(b, op_start, ops, params, nparams, lds_max_qubits);
+  if (rc) {
+    dm_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return DM_OK;
+}
+
+int dm_batch_run(dm_batch *b, int use_graph, double *probs_out, double *circuit_us, double *total_ms) {
+  if (!b) return fail(DM_ERR_ARG, "null batch");
+  if (use_graph && !b->exec) {
+    HIP_TRY(hipStreamBeginCapture(b->master, hipStreamCaptureModeThreadLocal));
+    const int rc = b->dtype == DM_FP64 ? enqueue_batch<double>(b) : enqueue_batch<float>(b);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->master, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(DM_ERR_HIP, "batch capture: %s", hipGetErrorString(e));
+    b->graph = g;
+    HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  HIP_TRY(hipEventRecord(b->ev0, b->master));
+  if (use_graph) {
+    HIP_TRY(hipGraphLaunch(b->exec, b->master));
+  } else {
+    const int rc = b->dtype == DM_FP64 ? enqueue_batch<double>(b) : enqueue_batch<float>(b);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipEventRecord(b->ev1, b->master));
+  HIP_TRY(hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  if (total_ms) *total_ms = ms;
+  if (probs_out) HIP_TRY(hipMemcpy(probs_out, b->d_probs, b->nprobs * sizeof(double), hipMemcpyDeviceToHost));
+  if (circuit_us) {
+    std::vector<uint64_t> st(2 * (size_t)b->ncirc);
+    HIP_TRY(hipMemcpy(st.data(), b->d_stamps, st.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    // stamps are stored small circuits first, then HBM circuits; report in circuit order
+    std::vector<int> order(b->small);
+    order.insert(order.end(), b->large.begin(), b->large.end());
+    for (size_t j = 0; j < order.size(); ++j)
+      circuit_us[order[j]] = (double)(st[2 * j + 1] - st[2 * j]) / b->clock_khz * 1e3;
+  }
+  return DM_OK;
 }
 
 int dm_synchronize(dm_state *h) {

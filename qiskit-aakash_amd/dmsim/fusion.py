@@ -157,6 +157,27 @@ def plan(gates, n, kmax=4, method='greedy', cost=None):
     return blocks
 
 
+def compose_single_qubit_runs(gates):
+    """Same gates with each run of 1-qubit PTMs on one qubit (no two-qubit gate
+    on that qubit in between) folded into one PTM; gates on different qubits
+    commute, so each folded PTM is emitted just before the next two-qubit
+    gate on its qubit (or at the end)."""
+    pending = {}
+    out = []
+    for g in gates:
+        if len(g.qubits) == 1:
+            q = g.qubits[0]
+            pending[q] = g.R @ pending[q] if q in pending else g.R
+            continue
+        for q in g.qubits:
+            if q in pending:
+                out.append(NoisyGate((q,), pending.pop(q), 'u3'))
+        out.append(g)
+    for q in sorted(pending):
+        out.append(NoisyGate((q,), pending[q], 'u3'))
+    return out
+
+
 def _subop(g):
     if g.kind == 'u3':
         return (native.SUB_1Q, list(g.qubits), ptm.rows3(g.R))
@@ -187,7 +208,7 @@ def add_block(prog, gates, qs, mem, mode='seq', tile=6):
     if len(qs) == 1:
         prog.one_qubit(qs[0], block_ptm(sub, qs))
     elif mode == 'seq' and n >= tile:
-        prog.fused(tile, qs, [_subop(g) for g in sub])
+        prog.fused(tile, qs, [_subop(g) for g in compose_single_qubit_runs(sub)])
     elif len(qs) == 2:
         # block_ptm's pair index is i_qs0 + 4 i_qs1: qs[1] is the leading qubit
         prog.two_qubit(qs[1], qs[0], block_ptm(sub, qs).reshape(16, 16))

@@ -54,6 +54,11 @@ SIGNATURES = {
                                         c_float_p, c_float_p]),
     'dm_mlp_train': (ctypes.c_int, [c_float_p, ctypes.c_int, ctypes.c_int, c_float_p, c_float_p, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, c_float_p]),
+    'dm_batch_create': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, c_int_p, ctypes.POINTER(ctypes.c_int64),
+                                       ctypes.c_void_p, c_double_p, ctypes.c_int64, c_double_p, ctypes.c_int,
+                                       ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    'dm_batch_run': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_double_p, c_double_p, c_double_p]),
+    'dm_batch_destroy': (ctypes.c_int, [ctypes.c_void_p]),
     'dm_synchronize': (ctypes.c_int, [ctypes.c_void_p]),
     'dm_profile': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     'dm_profile_read': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_u64_p, c_double_p]),
