@@ -175,6 +175,140 @@ def cpu_baseline(n_target, G, seed, budget_s=20.0, n_sample=13):
                       '%.3f gates/s; scaled by 4^-%d to %d qubits' % (done, n, el, rate, n_target - n, n_target)}
 
 
+def kernels_mode(n=16, reps=3):
+    """One short run of each kernel family at n qubits (for rocprofv3 counter
+    passes): a diagonal memory-noise pass (reads and writes every coefficient
+    once: the traffic calibration), plain 1q / CX passes and fused passes."""
+    from dmsim.program import Program
+    out = {}
+    noisy = fusion.noisy_gates(workloads.local_u3_cx(n, 200, 0), 1e-3)
+    fused, blocks = fusion.fused_program(noisy, n, 4, 'greedy', None, 'seq')
+    big = max(blocks, key=lambda b: len(b[1]))
+    for prec in ('fp64', 'fp32'):
+        st = native.DeviceState(n, prec)
+        zero_state(st, n)
+        progs = {
+            'calib_diag': (lambda p: p.memory_noise(0.99, 0.5, 0.98)),
+            'plain_1q_high': (lambda p: p.one_qubit(0, noisy[0].R if noisy[0].kind == 'u3' else np.eye(4))),
+            'plain_1q_low': (lambda p: p.one_qubit(n - 1, np.eye(4))),
+            'plain_cx': (lambda p: p.cx(3, 4, None)),
+            'fused_block': (lambda p: fusion.add_block(p, noisy, big[0], big[1], 'seq')),
+        }
+        for name, fill in progs.items():
+            prog = Program(n)
+            for _ in range(reps):
+                fill(prog)
+            prog.run(st)
+            st.synchronize()
+            st.profile(True)
+            st.profile_reset()
+            prog.run(st)
+            tot = sum(st.profile_read(f)[1] for f in native.FAMILIES)
+            st.profile(False)
+            out['%s_%s_ms' % (name, prec)] = tot / reps
+        st.close()
+    return out
+
+
+def batch_mode(args):
+    from dmsim import batch
+    specs = workloads.batch(512, args.seed)
+    lines = {}
+    for prec in ('fp64', 'fp32'):
+        (lmax, thr), tune = batch.autotune(specs, prec)
+        b = batch.from_workload(specs, prec, lds_max_qubits=lmax, lds_threads=thr)
+        for _ in range(args.warmup):
+            b.run(True)
+        t0 = time.perf_counter()
+        us_all = []
+        for _ in range(args.steps):
+            _, us, ms = b.run(True)
+            us_all.append(us)
+        wall = time.perf_counter() - t0
+        b.close()
+        us = np.concatenate(us_all)
+        lines[prec] = {'circuits_per_s': 512 * args.steps / wall, 'p50_us': float(np.percentile(us, 50)),
+                       'p99_us': float(np.percentile(us, 99)), 'lds_max_qubits': lmax, 'lds_threads': thr,
+                       'autotune_ms': tune.get('trials_ms')}
+        log('%s: %s' % (prec, json.dumps(lines[prec])))
+    return {'metric': 'circuits/sec, 512 noisy circuits of 6-12 qubits (one hipGraph launch)',
+            'value': lines['fp64']['circuits_per_s'], 'unit': 'circuits/s', 'n_gpus': 1, 'steps': args.steps,
+            'warmup': args.warmup, 'higher_is_better': True, 'dtype': 'f64', 'data': 'synthetic',
+            'config': {'workload': '512 circuits, widths U{6..12}, gates U{50..300} (U3 0.7 / CX 0.3), decay and '
+                                   'depolarisation rates logU[1e-4,1e-2], barrier after every gate, seed %d'
+                                   % args.seed, 'fp64': lines['fp64'], 'fp32': lines['fp32']}}
+
+
+def precision_mode(args):
+    """17 qubits, FP32 and FP64 states resident together, the same fused
+    program (depolarising 5e-4 per gate) on both; errors on the device."""
+    n = 17
+    gates = workloads.matching_layers(n, args.layers, args.seed)
+    noisy = fusion.noisy_gates(gates, 5e-4)
+    prog, blocks = fusion.fused_program(noisy, n, 4, 'greedy', None, 'seq')
+    res = {}
+    states = {}
+    for prec in ('fp64', 'fp32'):
+        st = native.DeviceState(n, prec)
+        zero_state(st, n)
+        st.synchronize()
+        t0 = time.perf_counter()
+        prog.run(st)
+        st.synchronize()
+        res[prec + '_gates_per_s'] = len(noisy) / (time.perf_counter() - t0)
+        states[prec] = st
+    cmp = states['fp32'].compare(states['fp64'])
+    fid = cmp['dot'] / math.sqrt(cmp['norm2_self'] * cmp['norm2_other'])
+    m32 = [states['fp32'].probabilities([q], 'Z')[0] for q in range(n)]
+    m64 = [states['fp64'].probabilities([q], 'Z')[0] for q in range(n)]
+    for st in states.values():
+        st.close()
+    res.update({'max_abs_coeff_err': cmp['max_abs_diff'], 'max_abs_coeff_err_x2n': cmp['max_abs_diff'] * 2.0 ** n,
+                'fidelity_overlap': fid, 'purity_fp64': cmp['norm2_other'] * 2.0 ** n,
+                'purity_fp32': cmp['norm2_self'] * 2.0 ** n,
+                'max_marginal_err': float(np.max(np.abs(np.array(m32) - np.array(m64)))),
+                'marginals_fp64': m64, 'gates': len(noisy), 'passes': len(prog)})
+    log(json.dumps(res))
+    return {'metric': '17-qubit FP32 vs FP64 accuracy and throughput (fused)', 'value': res['fp32_gates_per_s'],
+            'unit': 'gates/s', 'n_gpus': 1, 'higher_is_better': True, 'dtype': 'f32', 'data': 'synthetic',
+            'config': dict(res, workload='17q, %d layers of U3 on every qubit + CX on a random perfect matching, '
+                                          'depolarising 5e-4 per gate, seed %d' % (args.layers, args.seed))}
+
+
+def thermal_mode(args):
+    """14 qubits, 30 brickwork layers + QFT, thermal relaxation (T1 = 50 us,
+    T2 = 70 us, 50 ns gates) after every gate, partial Z measure of qubits 0-3,
+    through the backend (execute) in FP64."""
+    from dmsim import BasicAer, execute
+    n = 14
+    g, f = workloads.thermal_factors(50e-6, 70e-6, 50e-9)
+    gates = workloads.brickwork_qft(n, 30, args.seed)
+    qc = workloads.to_circuit(n, gates, barrier_after_each=True, measure=('partial', [0, 1, 2, 3], 'Z'))
+    be = BasicAer.get_backend('dm_simulator')
+    out = {}
+    for fusion_mode in ('none', 'greedy'):
+        t0 = time.perf_counter()
+        r = execute(qc, be, decay_factor=g, decoherence_factor=f, thermal_factor=1.0,
+                    compute_densitymatrix=False, fusion=fusion_mode).result()
+        wall = time.perf_counter() - t0
+        res = r['results'][0]
+        out[fusion_mode] = {'wall_s': wall, 'running_time_taken': res['running_time_taken'],
+                            'levels': res['number_of_clock_cycles'],
+                            'partial_probability': res['data']['partial_probability'],
+                            'trace': float(res['data']['coeffmatrix'][0] * 2 ** n)}
+        log('%s: %s' % (fusion_mode, json.dumps({k: v for k, v in out[fusion_mode].items()
+                                                  if k != 'partial_probability'})))
+    pp = out['none']['partial_probability']
+    pq = out['greedy']['partial_probability']
+    return {'metric': '14-qubit thermal brickwork+QFT, gates/s (FP64, execute)',
+            'value': len(gates) / out['none']['running_time_taken'], 'unit': 'gates/s', 'n_gpus': 1,
+            'higher_is_better': True, 'dtype': 'f64', 'data': 'synthetic',
+            'config': {'workload': '14q, 30 brickwork layers + QFT, T1=50us T2=70us dt=50ns, barrier after every '
+                                   'gate, partial Z on q0-3', 'plain': out['none'], 'fused': out['greedy'],
+                       'fused_gates_per_s': len(gates) / out['greedy']['running_time_taken'],
+                       'max_partial_prob_diff_fused_vs_plain': max(abs(pp[k] - pq[k]) for k in pp)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -187,11 +321,22 @@ def main():
     ap.add_argument('--no-accuracy', action='store_true')
     ap.add_argument('--fusion', default='auto', choices=['auto', 'greedy', 'learned'])
     ap.add_argument('--quick', action='store_true', help='time only the plain fp64 and headline paths')
+    ap.add_argument('--config', default='headline', choices=['headline', 'batch', 'precision', 'thermal'])
+    ap.add_argument('--layers', type=int, default=40, help='precision config: layers')
+    ap.add_argument('--kernels', action='store_true', help='short per-kernel run (for rocprofv3 counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU: the path has no multi-GPU form in this build (--gpus %d refused)'
               % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.kernels:
+        print(json.dumps(kernels_mode(args.qubits)), flush=True)
+        return
+    if args.config != 'headline':
+        fn = {'batch': batch_mode, 'precision': precision_mode, 'thermal': thermal_mode}[args.config]
+        print(json.dumps(fn(args)), flush=True)
+        return
 
     n, G = args.qubits, args.gates
     gates = workloads.local_u3_cx(n, G, args.seed)

@@ -121,6 +121,11 @@ int dm_trace_purity(dm_state *h, double *tr, double *purity);
  * (dm_simulator.py:1277-1282). */
 int dm_overlap(dm_state *h, const double *other_host, double *out);
 
+/* Two states of the same register (any storage types) compared on the device
+ * in FP64: max |a-b|, sum a*b, sum a^2, sum b^2 (state fidelity of the
+ * reference's compare option is 2^n sum a*b, dm_simulator.py:1277-1282). */
+int dm_compare(dm_state *a, dm_state *b, double *max_abs_diff, double *dot, double *norm2_a, double *norm2_b);
+
 /* 2^n x 2^n complex128 density matrix (row-major, interleaved re/im), qubit 0
  * the most significant bit of row and column.  Replaces _compute_densitymatrix
  * (dm_simulator.py:1198-1255). */
@@ -141,7 +146,7 @@ int dm_to_dense(dm_state *h, double *out_c128);
  *   DM_OP_FUSED         one HBM pass applying several operations to tiles
  *                       of 4^m coefficients staged in LDS; a variable-length
  *                       record of values (integers stored exactly):
- *                         m (5 or 6), nq, q_1 .. q_nq  -- qubits the tile
+ *                         m (5, 6 or 7), nq, q_1 .. q_nq  -- qubits the tile
  *                           holds; it is completed with the qubits of lowest
  *                           stride, and must hold qubit n-1
  *                         nsub, then nsub sub-operations, each

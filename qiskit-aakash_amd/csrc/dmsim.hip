@@ -587,7 +587,7 @@ struct TileOp {
 struct TileGeom {
   int m;                  // tile digits (== template M)
   int w;                  // leading digits with tpos[j] == j
-  int tpos[8];
+  int tpos[8];  // up to 7 tile digits
   int nr;
   int rpos[kMaxQubits];
   uint64_t tiles;
@@ -655,9 +655,19 @@ __device__ __forceinline__ void tile_dense(T *s, const TileOp &op, const T *__re
 }
 
 template <typename T, int M>
-__global__ __launch_bounds__(kBlock) void k_tile_prog(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
+__device__ __forceinline__ uint64_t tile_offset(int t, int w, int lomask, const TileGeom &geo) {
+  uint64_t off = (uint64_t)(t & lomask);
+  for (int j = w; j < M; ++j) off |= (uint64_t)((t >> (2 * j)) & 3) << (2 * geo.tpos[j]);
+  return off;
+}
+
+template <typename T, int M, bool DENSE, bool GEN2Q, int NT>
+__global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
                                                       int nops, const T *__restrict__ prm) {
   constexpr int E = 1 << (2 * M);
+  constexpr int QPT = (E / 4 + NT - 1) / NT;  // quads per thread (4 for M=6, 1 for M=5)
+  constexpr int G1 = (E / 4 + NT - 1) / NT;   // 1q groups per thread
+  using Q = Pack<T, 4>;
   __shared__ __align__(32) T s[E];
   const int tid = threadIdx.x;
   const int w = geo.w;
@@ -668,21 +678,41 @@ __global__ __launch_bounds__(kBlock) void k_tile_prog(T *__restrict__ a, TileGeo
   // workgroups of the same XCD: logical id = (b % 8) * (grid / 8) + b / 8.
   const uint64_t g8 = gridDim.x / 8;
   const uint64_t lid = (gridDim.x % 8 == 0) ? (blockIdx.x % 8) * g8 + blockIdx.x / 8 : blockIdx.x;
-  for (uint64_t tile = lid; tile < geo.tiles; tile += gridDim.x) {
-    uint64_t base = 0;
-    {
-      uint64_t r = tile;
-      for (int j = 0; j < geo.nr; ++j) {
-        base |= (r & 3ull) << (2 * geo.rpos[j]);
-        r >>= 2;
-      }
+  auto tile_base = [&](uint64_t tile) {
+    uint64_t base = 0, r = tile;
+    for (int j = 0; j < geo.nr; ++j) {
+      base |= (r & 3ull) << (2 * geo.rpos[j]);
+      r >>= 2;
     }
-    // stage: quads of 4 consecutive tile indices are 4 consecutive coefficients
-    for (int c = tid; c < E / 4; c += kBlock) {
-      const int t = 4 * c;
-      uint64_t off = (uint64_t)(t & lomask);
-      for (int j = w; j < M; ++j) off |= (uint64_t)((t >> (2 * j)) & 3) << (2 * geo.tpos[j]);
-      *reinterpret_cast<Pack<T, 4> *>(&s[t]) = ldp<Pack<T, 4>>(a + base + off);
+    return base;
+  };
+  // quads of 4 consecutive tile indices are 4 consecutive coefficients
+  Q buf[QPT];
+  uint64_t tile = lid;
+  if (tile < geo.tiles) {
+    const uint64_t base = tile_base(tile);
+#pragma unroll
+    for (int i = 0; i < QPT; ++i) {
+      const int c = tid + i * NT;
+      if (c < E / 4) buf[i] = ldp<Q>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo));
+    }
+  }
+  for (; tile < geo.tiles; tile += gridDim.x) {
+    const uint64_t base = tile_base(tile);
+#pragma unroll
+    for (int i = 0; i < QPT; ++i) {
+      const int c = tid + i * NT;
+      if (c < E / 4) *reinterpret_cast<Q *>(&s[4 * c]) = buf[i];
+    }
+    // prefetch the next tile while this one is computed
+    const uint64_t next = tile + gridDim.x;
+    if (next < geo.tiles) {
+      const uint64_t nb = tile_base(next);
+#pragma unroll
+      for (int i = 0; i < QPT; ++i) {
+        const int c = tid + i * NT;
+        if (c < E / 4) buf[i] = ldp<Q>(a + nb + tile_offset<T, M>(4 * c, w, lomask, geo));
+      }
     }
     __syncthreads();
     for (int i = 0; i < nops; ++i) {
@@ -690,7 +720,6 @@ __global__ __launch_bounds__(kBlock) void k_tile_prog(T *__restrict__ a, TileGeo
       const T *pv = prm + op.param;
       if (op.kind == TOP_1Q) {
         const int j = op.d[0];
-        const int st = 1 << (2 * j);
         T r[3][4];
 #pragma unroll
         for (int x = 0; x < 3; ++x)
@@ -698,37 +727,59 @@ __global__ __launch_bounds__(kBlock) void k_tile_prog(T *__restrict__ a, TileGeo
           for (int y = 0; y < 4; ++y) r[x][y] = pv[4 * x + y];
         if (j == 0) {
           // the four components of a group are adjacent: one vector LDS access
-          for (int g = tid; g < E / 4; g += kBlock) {
-            Pack<T, 4> x = *reinterpret_cast<const Pack<T, 4> *>(&s[4 * g]);
-            Pack<T, 4> y;
-            y.v[0] = x.v[0];
+          Q x[G1];
 #pragma unroll
-            for (int q = 0; q < 3; ++q)
-              y.v[q + 1] = r[q][0] * x.v[0] + r[q][1] * x.v[1] + r[q][2] * x.v[2] + r[q][3] * x.v[3];
-            *reinterpret_cast<Pack<T, 4> *>(&s[4 * g]) = y;
+          for (int q = 0; q < G1; ++q) {
+            const int g = tid + q * NT;
+            if (g < E / 4) x[q] = *reinterpret_cast<const Q *>(&s[4 * g]);
           }
-          __syncthreads();
-          continue;
+#pragma unroll
+          for (int q = 0; q < G1; ++q) {
+            const int g = tid + q * NT;
+            if (g >= E / 4) continue;
+            Q y;
+            y.v[0] = x[q].v[0];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+              y.v[c + 1] = r[c][0] * x[q].v[0] + r[c][1] * x[q].v[1] + r[c][2] * x[q].v[2] + r[c][3] * x[q].v[3];
+            *reinterpret_cast<Q *>(&s[4 * g]) = y;
+          }
+        } else {
+          // all of this thread's groups are read before any is written
+          const int st = 1 << (2 * j);
+          T x[G1][4];
+          int bb[G1];
+#pragma unroll
+          for (int q = 0; q < G1; ++q) {
+            const int g = tid + q * NT;
+            bb[q] = ins_zero(g, j);
+            if (g < E / 4) {
+#pragma unroll
+              for (int c = 0; c < 4; ++c) x[q][c] = s[bb[q] + c * st];
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < G1; ++q) {
+            const int g = tid + q * NT;
+            if (g >= E / 4) continue;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+              s[bb[q] + (c + 1) * st] =
+                  r[c][0] * x[q][0] + r[c][1] * x[q][1] + r[c][2] * x[q][2] + r[c][3] * x[q][3];
+          }
         }
-        for (int g = tid; g < E / 4; g += kBlock) {
-          const int b = ins_zero(g, j);
-          const T x0 = s[b], x1 = s[b + st], x2 = s[b + 2 * st], x3 = s[b + 3 * st];
-          s[b + st] = r[0][0] * x0 + r[0][1] * x1 + r[0][2] * x2 + r[0][3] * x3;
-          s[b + 2 * st] = r[1][0] * x0 + r[1][1] * x1 + r[1][2] * x2 + r[1][3] * x3;
-          s[b + 3 * st] = r[2][0] * x0 + r[2][1] * x1 + r[2][2] * x2 + r[2][3] * x3;
-        }
-      } else if (op.kind == TOP_CX || op.kind == TOP_2Q) {
+      } else if (op.kind == TOP_CX || (GEN2Q && op.kind == TOP_2Q)) {
         const int jc = op.d[0], jt = op.d[1];
         const int jl = min(jc, jt), jh = max(jc, jt);
         const int sc = 1 << (2 * jc), stt = 1 << (2 * jt);
-        for (int g = tid; g < E / 16; g += kBlock) {
+        for (int g = tid; g < E / 16; g += NT) {
           const int b = ins_zero(ins_zero(g, jl), jh);
           T v[4][4];
 #pragma unroll
           for (int c = 0; c < 4; ++c)
 #pragma unroll
             for (int t = 0; t < 4; ++t) v[c][t] = s[b + c * sc + t * stt];
-          if (op.kind == TOP_CX) {
+          if (!GEN2Q || op.kind == TOP_CX) {
             Scale16<T> sv;
 #pragma unroll
             for (int q = 0; q < 16; ++q) sv.s[q] = op.k ? pv[q] : T(1);
@@ -748,16 +799,15 @@ __global__ __launch_bounds__(kBlock) void k_tile_prog(T *__restrict__ a, TileGeo
             }
           }
         }
-      } else if (op.kind == TOP_DENSE) {
+      } else if (DENSE && op.kind == TOP_DENSE) {
         if (M == 6) tile_dense<T, M>(s, op, pv);
       }
       __syncthreads();
     }
-    for (int c = tid; c < E / 4; c += kBlock) {
-      const int t = 4 * c;
-      uint64_t off = (uint64_t)(t & lomask);
-      for (int j = w; j < M; ++j) off |= (uint64_t)((t >> (2 * j)) & 3) << (2 * geo.tpos[j]);
-      stp(a + base + off, *reinterpret_cast<const Pack<T, 4> *>(&s[t]));
+#pragma unroll
+    for (int i = 0; i < QPT; ++i) {
+      const int c = tid + i * NT;
+      if (c < E / 4) stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), *reinterpret_cast<const Q *>(&s[4 * c]));
     }
     __syncthreads();
   }
@@ -806,6 +856,41 @@ __global__ __launch_bounds__(kBlock) void k_dot(const T *__restrict__ a, const d
     s += (double)a[i] * b[i];
   s = block_sum(s);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// per block: max |a-b|, sum a*b, sum a*a, sum b*b
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kBlock) void k_diff(const TA *__restrict__ a, const TB *__restrict__ b, uint64_t N,
+                                                 double *part) {
+  double mx = 0.0, ab = 0.0, aa = 0.0, bb = 0.0;
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += step) {
+    const double x = (double)a[i], y = (double)b[i];
+    mx = fmax(mx, fabs(x - y));
+    ab += x * y;
+    aa += x * x;
+    bb += y * y;
+  }
+  __shared__ double s_m[kBlock];
+  s_m[threadIdx.x] = mx;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) s_m[threadIdx.x] = fmax(s_m[threadIdx.x], s_m[threadIdx.x + off]);
+    __syncthreads();
+  }
+  const double m = s_m[0];
+  __syncthreads();
+  ab = block_sum(ab);
+  __syncthreads();
+  aa = block_sum(aa);
+  __syncthreads();
+  bb = block_sum(bb);
+  if (threadIdx.x == 0) {
+    part[4 * blockIdx.x] = m;
+    part[4 * blockIdx.x + 1] = ab;
+    part[4 * blockIdx.x + 2] = aa;
+    part[4 * blockIdx.x + 3] = bb;
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_sum_parts(const double *part, int nparts, double *out) {
@@ -1097,10 +1182,14 @@ struct dm_state {
 
 namespace {
 
+// One thread per work unit: for streaming passes a full grid measured faster
+// on gfx950 than a capped grid-stride launch.  The kernels keep 64-bit
+// grid-stride loops; the cap (2^22 workgroups = 2^30 work-items) keeps a
+// launch below the 2^32 work-item limit of one grid dimension.
 int grid_for(const dm_state *h, uint64_t units) {
+  (void)h;
   const uint64_t want = (units + kBlock - 1) / kBlock;
-  const uint64_t cap = (uint64_t)h->cu_count * 64;  // >= 16 waves per SIMD worth of blocks
-  return (int)std::max<uint64_t>(1, std::min(want, cap));
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>(want, 1ull << 22));
 }
 
 int ensure(void **p, size_t *cap, size_t bytes) {
@@ -1602,7 +1691,7 @@ int parse_fused(const dm_state *h, const double *p, int64_t avail, int opi, Fuse
     return true;
   };
   int m = 0, nq = 0;
-  if (!next_int(&m) || (m != 5 && m != 6)) return fail(DM_ERR_ARG, "op %d: fused tile size must be 5 or 6", opi);
+  if (!next_int(&m) || m < 5 || m > 7) return fail(DM_ERR_ARG, "op %d: fused tile size must be 5, 6 or 7", opi);
   if (h->n < m) return fail(DM_ERR_ARG, "op %d: fused tile of %d qubits on %d qubits", opi, m, h->n);
   if (!next_int(&nq) || nq < 0 || nq > m) return fail(DM_ERR_ARG, "op %d: bad fused qubit count", opi);
   bool intile[kMaxQubits] = {false};
@@ -1690,15 +1779,29 @@ int parse_fused(const dm_state *h, const double *p, int64_t avail, int opi, Fuse
 template <typename T>
 int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T *d_vals) {
   ProfScope ps(h, 9);
-  int grid = (int)std::min<uint64_t>(fp.geo.tiles, (uint64_t)h->cu_count * 8);
+  const int per_cu = fp.m == 7 ? 2 : 8;  // resident workgroups per CU (LDS-limited for m = 7)
+  int grid = (int)std::min<uint64_t>(fp.geo.tiles, (uint64_t)h->cu_count * per_cu);
   if (grid >= 8) grid -= grid % 8;  // whole XCD rounds (see k_tile_prog)
   const int nops = (int)fp.ops.size();
-  if (fp.m == 6)
-    hipLaunchKernelGGL((k_tile_prog<T, 6>), dim3(grid), dim3(kBlock), 0, h->stream, static_cast<T *>(h->d_a),
-                       fp.geo, d_ops, nops, d_vals);
-  else
-    hipLaunchKernelGGL((k_tile_prog<T, 5>), dim3(grid), dim3(kBlock), 0, h->stream, static_cast<T *>(h->d_a),
-                       fp.geo, d_ops, nops, d_vals);
+  bool dense = false, gen = false;
+  for (const TileOp &op : fp.ops) {
+    dense = dense || op.kind == TOP_DENSE;
+    gen = gen || op.kind == TOP_2Q;
+  }
+  T *a = static_cast<T *>(h->d_a);
+  // one variant per feature set keeps the register footprint of the common
+  // 1q + CX passes small; 7-qubit tiles (64 / 128 KB of LDS) use 512 threads
+#define TILE_LAUNCH(MM, D, G, NTH)                                                                              \
+  hipLaunchKernelGGL((k_tile_prog<T, MM, D, G, NTH>), dim3(grid), dim3(NTH), 0, h->stream, a, fp.geo, d_ops, nops, \
+                     d_vals)
+  if (fp.m == 7 && gen) TILE_LAUNCH(7, false, true, 512);
+  else if (fp.m == 7) TILE_LAUNCH(7, false, false, 512);
+  else if (fp.m == 6 && dense) TILE_LAUNCH(6, true, true, 256);
+  else if (fp.m == 6 && gen) TILE_LAUNCH(6, false, true, 256);
+  else if (fp.m == 6) TILE_LAUNCH(6, false, false, 256);
+  else if (gen) TILE_LAUNCH(5, false, true, 256);
+  else TILE_LAUNCH(5, false, false, 256);
+#undef TILE_LAUNCH
   return last_launch("k_tile_prog");
 }
 
@@ -2109,6 +2212,31 @@ static int batch_create_t(dm_batch *b, const int64_t *op_start, const dm_op *ops
   return DM_OK;
 }
 
+template <typename TA, typename TB>
+static int diff_t(dm_state *a, dm_state *b, double *out4) {
+  const int grid = std::min(grid_for(a, a->N), 2048);
+  if (int rc = reserve_red(a, 4 * (size_t)grid)) return rc;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  hipLaunchKernelGGL((k_diff<TA, TB>), dim3(grid), dim3(kBlock), 0, a->stream, static_cast<const TA *>(a->d_a),
+                     static_cast<const TB *>(b->d_a), a->N, a->d_red);
+  if (int rc = last_launch("k_diff")) return rc;
+  std::vector<double> part(4 * (size_t)grid);
+  HIP_TRY(hipMemcpyAsync(part.data(), a->d_red, part.size() * sizeof(double), hipMemcpyDeviceToHost, a->stream));
+  HIP_TRY(hipStreamSynchronize(a->stream));
+  double m = 0, ab = 0, aa = 0, bb = 0;
+  for (int i = 0; i < grid; ++i) {
+    m = std::max(m, part[4 * i]);
+    ab += part[4 * i + 1];
+    aa += part[4 * i + 2];
+    bb += part[4 * i + 3];
+  }
+  out4[0] = m;
+  out4[1] = ab;
+  out4[2] = aa;
+  out4[3] = bb;
+  return DM_OK;
+}
+
 #define DISPATCH(h, fn, ...) ((h)->dtype == DM_FP64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
 
 extern "C" {
@@ -2489,6 +2617,24 @@ int dm_batch_run(dm_batch *b, int use_graph, double *probs_out, double *circuit_
     for (size_t j = 0; j < order.size(); ++j)
       circuit_us[order[j]] = (double)(st[2 * j + 1] - st[2 * j]) / b->clock_khz * 1e3;
   }
+  return DM_OK;
+}
+
+int dm_compare(dm_state *a, dm_state *b, double *max_abs_diff, double *dot, double *norm2_a, double *norm2_b) {
+  if (int rc = check_handle(a)) return rc;
+  if (int rc = check_handle(b)) return rc;
+  if (a->n != b->n) return fail(DM_ERR_ARG, "states of %d and %d qubits", a->n, b->n);
+  double o[4];
+  int rc;
+  if (a->dtype == DM_FP64 && b->dtype == DM_FP64) rc = diff_t<double, double>(a, b, o);
+  else if (a->dtype == DM_FP64) rc = diff_t<double, float>(a, b, o);
+  else if (b->dtype == DM_FP64) rc = diff_t<float, double>(a, b, o);
+  else rc = diff_t<float, float>(a, b, o);
+  if (rc) return rc;
+  if (max_abs_diff) *max_abs_diff = o[0];
+  if (dot) *dot = o[1];
+  if (norm2_a) *norm2_a = o[2];
+  if (norm2_b) *norm2_b = o[3];
   return DM_OK;
 }
 

@@ -36,18 +36,9 @@ PARAMS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'costmode
 KINDS = ('plain1q', 'plaincx', 'seq', 'dense')
 
 
-def tile_run_digits(n, positions, m=TILE):
-    """w: how many lowest positions the tile holds (the tile is the block's
-    positions completed with the lowest other positions up to m)."""
-    inside = set(positions)
-    p = 0
-    while len(inside) < m and p < n:
-        inside.add(p)
-        p += 1
-    w = 0
-    while w in inside:
-        w += 1
-    return w
+def tile_run_digits(n, positions):
+    """w of the tile a sequential pass over these positions uses."""
+    return fusion.run_digits(n, positions, fusion.tile_for(n, [n - 1 - p for p in positions]))
 
 
 def features(kind, n, qubits, n1, n2, precision):
@@ -58,7 +49,13 @@ def features(kind, n, qubits, n1, n2, precision):
     x[4] = len(pos) / 6.0
     x[5] = math.log2(1 + n1) / 4.0
     x[6] = math.log2(1 + n2) / 4.0
-    x[7] = (tile_run_digits(n, pos) if kind in ('seq', 'dense') else (1 if min(pos) > 0 else 0)) / 6.0
+    if kind == 'seq':
+        w = tile_run_digits(n, pos)
+    elif kind == 'dense':
+        w = fusion.run_digits(n, pos, 6)
+    else:
+        w = 1 if min(pos) > 0 else 0
+    x[7] = w / 6.0
     x[8] = 1.0 if precision == 'fp64' else 0.0
     x[9] = n / 20.0
     x[10] = pos[0] / n

@@ -186,11 +186,38 @@ def _subop(g):
     return (native.SUB_2Q, list(g.qubits), g.R)
 
 
-def add_block(prog, gates, qs, mem, mode='seq', tile=6):
+def run_digits(n, positions, m):
+    """w: the number of lowest positions a tile of m digits holds when it is
+    the block's positions completed with the lowest other positions; a tile
+    reads contiguous runs of 4^w coefficients from HBM."""
+    inside = set(positions)
+    p = 0
+    while len(inside) < m and p < n:
+        inside.add(p)
+        p += 1
+    w = 0
+    while w in inside:
+        w += 1
+    return w
+
+
+def tile_for(n, qs):
+    """Tile size of a sequential pass over block ``qs``: 6 digits (32 KB of
+    LDS in FP64) unless that leaves runs of only 4^2 coefficients, which
+    stream at about two thirds of the rate of longer runs; then 7 digits
+    (512 threads, 128 KB in FP64) when the register allows it."""
+    pos = [n - 1 - q for q in qs]
+    if n >= 7 and run_digits(n, pos, 6) < 3 and run_digits(n, pos, 7) > run_digits(n, pos, 6):
+        return 7
+    return 6
+
+
+def add_block(prog, gates, qs, mem, mode='seq', tile=None):
     """Append one block to a Program.
 
     A single gate runs on its plain kernel.  Otherwise, with n >= 6:
-      mode 'seq'   -- one tile pass applying the gates one after another in LDS;
+      mode 'seq'   -- one tile pass applying the gates one after another in LDS
+                      (tile digits: ``tile``, or ``tile_for`` when None);
       mode 'dense' -- one tile pass applying the block PTM on the matrix cores
                       (3-4 qubits; 1-2 qubit blocks use the dense 16x16 kernel).
     """
@@ -205,6 +232,8 @@ def add_block(prog, gates, qs, mem, mode='seq', tile=6):
         return
     sub = [gates[i] for i in mem]
     n = prog.n
+    if tile is None:
+        tile = tile_for(n, qs) if n >= 6 else 6
     if len(qs) == 1:
         prog.one_qubit(qs[0], block_ptm(sub, qs))
     elif mode == 'seq' and n >= tile:
@@ -227,16 +256,17 @@ def plain_program(gates, n):
     return prog
 
 
-def fused_program(gates, n, kmax=4, method='greedy', cost=None, mode='seq'):
+def fused_program(gates, n, kmax=4, method='greedy', cost=None, mode='seq', tile=None):
     """Program of fused passes; with a cost model each block also gets the
-    cheaper of the sequential and dense forms."""
+    cheaper of the sequential and dense forms.  ``tile`` fixes the tile size
+    of sequential passes (default: ``tile_for``)."""
     blocks = plan(gates, n, kmax, method, cost)
     prog = Program(n)
     for qs, mem in blocks:
         m = mode
         if cost is not None and len(mem) > 1:
             m = cost.best_mode(n, qs, [gates[i] for i in mem])
-        add_block(prog, gates, qs, mem, m)
+        add_block(prog, gates, qs, mem, m, tile)
     return prog, blocks
 
 

@@ -45,6 +45,8 @@ SIGNATURES = {
     'dm_trace_purity': (ctypes.c_int, [ctypes.c_void_p, c_double_p, c_double_p]),
     'dm_overlap': (ctypes.c_int, [ctypes.c_void_p, c_double_p, c_double_p]),
     'dm_to_dense': (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
+    'dm_compare': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_double_p, c_double_p, c_double_p,
+                                  c_double_p]),
     'dm_run_program': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, c_double_p,
                                       ctypes.c_int64]),
     'dm_mlp_param_count': (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
@@ -238,6 +240,13 @@ class DeviceState:
         out = ctypes.c_double(0)
         check(self.lib.dm_overlap(self.h, _dp(b), ctypes.byref(out)), 'dm_overlap')
         return out.value
+
+    def compare(self, other):
+        """FP64 device comparison with another state of the same register."""
+        m, d, a2, b2 = (ctypes.c_double(0) for _ in range(4))
+        check(self.lib.dm_compare(self.h, other.h, ctypes.byref(m), ctypes.byref(d), ctypes.byref(a2),
+                                  ctypes.byref(b2)), 'dm_compare')
+        return {'max_abs_diff': m.value, 'dot': d.value, 'norm2_self': a2.value, 'norm2_other': b2.value}
 
     def to_dense(self):
         dim = 2 ** self.n

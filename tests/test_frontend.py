@@ -210,3 +210,17 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(native.library_path())
     for name in declared:
         assert hasattr(lib, name), name
+
+
+def test_fused_tile_choice():
+    """Sequential passes take 7-digit tiles only where 6 digits would leave
+    contiguous HBM runs of 4^2 coefficients."""
+    from dmsim import fusion
+    n = 16
+    assert fusion.run_digits(n, [15, 14, 13, 12], 6) == 2
+    assert fusion.run_digits(n, [15, 14, 13, 12], 7) == 3
+    assert fusion.run_digits(n, [0, 7], 6) == 5
+    assert fusion.tile_for(n, [0, 1, 2, 3]) == 7      # positions 12..15
+    assert fusion.tile_for(n, [0, 1, 2]) == 6         # m=6 already gives w=3
+    assert fusion.tile_for(n, [15, 14]) == 6          # block at the lowest positions
+    assert fusion.tile_for(6, [0, 1, 2, 3]) == 6      # register too small for 7

@@ -283,6 +283,31 @@ def test_fused_passes_match_plain(prec, mode, n, kmax):
     assert np.max(np.abs(got - want)) <= t
 
 
+@pytest.mark.parametrize('prec', PRECS)
+@pytest.mark.parametrize('tile', [5, 6, 7])
+def test_fused_tile_sizes_match_plain(prec, tile):
+    """Sequential tile passes of 5, 6 and 7 digits (7: 512 threads) against
+    one kernel per gate."""
+    from dmsim import fusion, workloads
+    n = 9
+    gates = fusion.noisy_gates(workloads.local_u3_cx(n, 150, 40 + tile), 1e-3)
+    rng = np.random.default_rng(tile)
+    a = rand_state(n, rng)
+    plain = fusion.plain_program(gates, n)
+    fused, blocks = fusion.fused_program(gates, n, 4, 'greedy', None, 'seq', tile)
+    assert any(len(qs) > 1 for qs, _ in blocks)
+    s1 = make(n, 'fp64', a)
+    plain.run(s1)
+    want = s1.download()
+    s1.close()
+    s2 = make(n, prec, a)
+    fused.run(s2)
+    got = s2.download()
+    s2.close()
+    t = 1e-14 * 2.0 ** -n * 50 if prec == 'fp64' else 2e-6 * 2.0 ** -n * 50
+    assert np.max(np.abs(got - want)) <= t
+
+
 def test_fused_pass_rejects_bad_records():
     from dmsim.program import Program
     st = native.DeviceState(6, 'fp64')
@@ -295,3 +320,20 @@ def test_fused_pass_rejects_bad_records():
     with pytest.raises(native.NativeError, match='tile size'):
         p.run(st)
     st.close()
+
+
+def test_compare_states_across_precisions():
+    n = 6
+    rng = np.random.default_rng(12)
+    a = rand_state(n, rng)
+    b = a + rng.normal(size=a.size) * 1e-9
+    s1 = make(n, 'fp64', a)
+    s2 = make(n, 'fp64', b)
+    s3 = make(n, 'fp32', a)
+    r = s1.compare(s2)
+    assert abs(r['max_abs_diff'] - np.max(np.abs(a - b))) <= 1e-18
+    assert abs(r['dot'] - np.dot(a, b)) <= 1e-15 * abs(np.dot(a, b))
+    r3 = s1.compare(s3)
+    assert r3['max_abs_diff'] <= 1e-7 * np.max(np.abs(a))
+    for s in (s1, s2, s3):
+        s.close()
