@@ -60,7 +60,6 @@ template <typename P, typename T>
 __device__ __forceinline__ void stp(T *p, const P &x) {
   *reinterpret_cast<P *>(p) = x;
 }
-
 // Insert a zero 2-bit digit at each of the ascending positions pos[0..m-1].
 __device__ __forceinline__ uint64_t insert_digits(uint64_t r, const int *pos, int m) {
   for (int j = 0; j < m; ++j) {
@@ -667,6 +666,8 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
   constexpr int E = 1 << (2 * M);
   constexpr int QPT = (E / 4 + NT - 1) / NT;  // quads per thread (4 for M=6, 1 for M=5)
   constexpr int G1 = (E / 4 + NT - 1) / NT;   // 1q groups per thread
+  constexpr int GCMAX = sizeof(T) == 8 ? 2 : 4;
+  constexpr int GC = G1 < GCMAX ? G1 : GCMAX;  // 1q groups per chunk
   using Q = Pack<T, 4>;
   __shared__ __align__(32) T s[E];
   const int tid = threadIdx.x;
@@ -725,47 +726,55 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
         for (int x = 0; x < 3; ++x)
 #pragma unroll
           for (int y = 0; y < 4; ++y) r[x][y] = pv[4 * x + y];
+        // A thread's groups go in chunks of GC: all loads of a chunk are
+        // issued before its stores (LDS latency overlaps), and the chunk
+        // bounds the registers the op holds.
         if (j == 0) {
           // the four components of a group are adjacent: one vector LDS access
-          Q x[G1];
 #pragma unroll
-          for (int q = 0; q < G1; ++q) {
-            const int g = tid + q * NT;
-            if (g < E / 4) x[q] = *reinterpret_cast<const Q *>(&s[4 * g]);
-          }
+          for (int q0 = 0; q0 < G1; q0 += GC) {
+            Q x[GC];
 #pragma unroll
-          for (int q = 0; q < G1; ++q) {
-            const int g = tid + q * NT;
-            if (g >= E / 4) continue;
-            Q y;
-            y.v[0] = x[q].v[0];
+            for (int q = 0; q < GC; ++q) {
+              const int g = tid + (q0 + q) * NT;
+              if (g < E / 4) x[q] = *reinterpret_cast<const Q *>(&s[4 * g]);
+            }
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-              y.v[c + 1] = r[c][0] * x[q].v[0] + r[c][1] * x[q].v[1] + r[c][2] * x[q].v[2] + r[c][3] * x[q].v[3];
-            *reinterpret_cast<Q *>(&s[4 * g]) = y;
-          }
-        } else {
-          // all of this thread's groups are read before any is written
-          const int st = 1 << (2 * j);
-          T x[G1][4];
-          int bb[G1];
+            for (int q = 0; q < GC; ++q) {
+              const int g = tid + (q0 + q) * NT;
+              if (g >= E / 4) continue;
+              Q y;
+              y.v[0] = x[q].v[0];
 #pragma unroll
-          for (int q = 0; q < G1; ++q) {
-            const int g = tid + q * NT;
-            bb[q] = ins_zero(g, j);
-            if (g < E / 4) {
-#pragma unroll
-              for (int c = 0; c < 4; ++c) x[q][c] = s[bb[q] + c * st];
+              for (int c = 0; c < 3; ++c)
+                y.v[c + 1] = r[c][0] * x[q].v[0] + r[c][1] * x[q].v[1] + r[c][2] * x[q].v[2] + r[c][3] * x[q].v[3];
+              *reinterpret_cast<Q *>(&s[4 * g]) = y;
             }
           }
+        } else {
+          const int st = 1 << (2 * j);
 #pragma unroll
-          for (int q = 0; q < G1; ++q) {
-            const int g = tid + q * NT;
-            if (g >= E / 4) continue;
+          for (int q0 = 0; q0 < G1; q0 += GC) {
+            T x[GC][4];
+            int bb[GC];
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-              s[bb[q] + (c + 1) * st] =
-                  r[c][0] * x[q][0] + r[c][1] * x[q][1] + r[c][2] * x[q][2] + r[c][3] * x[q][3];
+            for (int q = 0; q < GC; ++q) {
+              const int g = tid + (q0 + q) * NT;
+              bb[q] = ins_zero(g, j);
+              if (g < E / 4) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) x[q][c] = s[bb[q] + c * st];
+              }
+            }
+#pragma unroll
+            for (int q = 0; q < GC; ++q) {
+              const int g = tid + (q0 + q) * NT;
+              if (g >= E / 4) continue;
+#pragma unroll
+              for (int c = 0; c < 3; ++c)
+                s[bb[q] + (c + 1) * st] =
+                    r[c][0] * x[q][0] + r[c][1] * x[q][1] + r[c][2] * x[q][2] + r[c][3] * x[q][3];
+            }
           }
         }
       } else if (op.kind == TOP_CX || (GEN2Q && op.kind == TOP_2Q)) {
@@ -1776,12 +1785,24 @@ int parse_fused(const dm_state *h, const double *p, int64_t avail, int opi, Fuse
   return DM_OK;
 }
 
+// Workgroups of one tile-program variant that fit on a CU at once (registers,
+// LDS, waves), from the runtime's occupancy calculator; cached per variant.
+template <typename T, int MM, bool D, bool G, int NTH>
+int tile_resident_per_cu() {
+  static int cached = 0;
+  if (cached == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tile_prog<T, MM, D, G, NTH>, NTH, 0) != hipSuccess ||
+        nb < 1)
+      nb = 1;
+    cached = nb;
+  }
+  return cached;
+}
+
 template <typename T>
 int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T *d_vals) {
   ProfScope ps(h, 9);
-  const int per_cu = fp.m == 7 ? 2 : 8;  // resident workgroups per CU (LDS-limited for m = 7)
-  int grid = (int)std::min<uint64_t>(fp.geo.tiles, (uint64_t)h->cu_count * per_cu);
-  if (grid >= 8) grid -= grid % 8;  // whole XCD rounds (see k_tile_prog)
   const int nops = (int)fp.ops.size();
   bool dense = false, gen = false;
   for (const TileOp &op : fp.ops) {
@@ -1791,9 +1812,17 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
   T *a = static_cast<T *>(h->d_a);
   // one variant per feature set keeps the register footprint of the common
   // 1q + CX passes small; 7-qubit tiles (64 / 128 KB of LDS) use 512 threads
-#define TILE_LAUNCH(MM, D, G, NTH)                                                                              \
-  hipLaunchKernelGGL((k_tile_prog<T, MM, D, G, NTH>), dim3(grid), dim3(NTH), 0, h->stream, a, fp.geo, d_ops, nops, \
-                     d_vals)
+  // Persistent workgroups, exactly as many as are resident at once (a grid
+  // larger than that leaves a tail of workgroups running at low occupancy),
+  // in whole rounds of the 8 XCDs (see k_tile_prog).
+#define TILE_LAUNCH(MM, D, G, NTH)                                                                               \
+  do {                                                                                                           \
+    const int per_cu = tile_resident_per_cu<T, MM, D, G, NTH>();                                                \
+    int grid = (int)std::min<uint64_t>(fp.geo.tiles, (uint64_t)h->cu_count * per_cu);                            \
+    if (grid >= 8) grid -= grid % 8;                                                                             \
+    hipLaunchKernelGGL((k_tile_prog<T, MM, D, G, NTH>), dim3(grid), dim3(NTH), 0, h->stream, a, fp.geo, d_ops,   \
+                       nops, d_vals);                                                                        \
+  } while (0)
   if (fp.m == 7 && gen) TILE_LAUNCH(7, false, true, 512);
   else if (fp.m == 7) TILE_LAUNCH(7, false, false, 512);
   else if (fp.m == 6 && dense) TILE_LAUNCH(6, true, true, 256);
