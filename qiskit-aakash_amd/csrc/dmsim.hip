@@ -662,7 +662,7 @@ __device__ __forceinline__ uint64_t tile_offset(int t, int w, int lomask, const 
 
 template <typename T, int M, bool DENSE, bool GEN2Q, int NT>
 __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
-                                                      int nops, const T *__restrict__ prm) {
+                                                      int nops, const T *__restrict__ prm, int remap) {
   constexpr int E = 1 << (2 * M);
   constexpr int QPT = (E / 4 + NT - 1) / NT;  // quads per thread (4 for M=6, 1 for M=5)
   constexpr int G1 = (E / 4 + NT - 1) / NT;   // 1q groups per thread
@@ -674,11 +674,11 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
   const int w = geo.w;
   const int lomask = (1 << (2 * w)) - 1;
   // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
-  // L2).  Neighbouring tiles share HBM lines (runs of 4^w coefficients are
-  // shorter than a line when w is small), so consecutive tiles go to
-  // workgroups of the same XCD: logical id = (b % 8) * (grid / 8) + b / 8.
+  // L2).  With remap set (persistent grids over short runs, where
+  // neighbouring tiles share HBM lines) consecutive tiles go to workgroups of
+  // the same XCD: logical id = (b % 8) * (grid / 8) + b / 8.
   const uint64_t g8 = gridDim.x / 8;
-  const uint64_t lid = (gridDim.x % 8 == 0) ? (blockIdx.x % 8) * g8 + blockIdx.x / 8 : blockIdx.x;
+  const uint64_t lid = (remap && gridDim.x % 8 == 0) ? (blockIdx.x % 8) * g8 + blockIdx.x / 8 : blockIdx.x;
   auto tile_base = [&](uint64_t tile) {
     uint64_t base = 0, r = tile;
     for (int j = 0; j < geo.nr; ++j) {
@@ -1800,6 +1800,7 @@ int tile_resident_per_cu() {
   return cached;
 }
 
+
 template <typename T>
 int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T *d_vals) {
   ProfScope ps(h, 9);
@@ -1812,16 +1813,24 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
   T *a = static_cast<T *>(h->d_a);
   // one variant per feature set keeps the register footprint of the common
   // 1q + CX passes small; 7-qubit tiles (64 / 128 KB of LDS) use 512 threads
-  // Persistent workgroups, exactly as many as are resident at once (a grid
-  // larger than that leaves a tail of workgroups running at low occupancy),
-  // in whole rounds of the 8 XCDs (see k_tile_prog).
+  // Grid shape, from 16-qubit sweeps on MI355X:
+  //  - runs of >= 4^3 coefficients with several workgroups resident per CU:
+  //    one tile per workgroup in tile order (the hardware overlaps the
+  //    load, LDS and store phases of different workgroups);
+  //  - runs of 4^2 (the tiles of neighbouring indices share HBM lines) or a
+  //    single resident workgroup per CU (7-digit FP64 tiles): persistent
+  //    workgroups that prefetch their next tile, 8 (resp. 4) per resident
+  //    slot, with consecutive tiles kept on one XCD.
 #define TILE_LAUNCH(MM, D, G, NTH)                                                                               \
   do {                                                                                                           \
     const int per_cu = tile_resident_per_cu<T, MM, D, G, NTH>();                                                \
-    int grid = (int)std::min<uint64_t>(fp.geo.tiles, (uint64_t)h->cu_count * per_cu);                            \
-    if (grid >= 8) grid -= grid % 8;                                                                             \
-    hipLaunchKernelGGL((k_tile_prog<T, MM, D, G, NTH>), dim3(grid), dim3(NTH), 0, h->stream, a, fp.geo, d_ops,   \
-                       nops, d_vals);                                                                        \
+    const bool stream_tiles = fp.geo.w >= 3 && per_cu >= 2;                                                      \
+    const uint64_t slots = (uint64_t)h->cu_count * per_cu * (per_cu >= 2 ? 8 : 4);                              \
+    uint64_t g = stream_tiles ? fp.geo.tiles : std::min<uint64_t>(fp.geo.tiles, slots);                          \
+    if (g >= 8) g -= g % 8;                                                                                      \
+    g = std::min<uint64_t>(g, 1ull << 30);                                                                       \
+    hipLaunchKernelGGL((k_tile_prog<T, MM, D, G, NTH>), dim3((unsigned)g), dim3(NTH), 0, h->stream, a, fp.geo,   \
+                       d_ops, nops, d_vals, stream_tiles ? 0 : 1);                                               \
   } while (0)
   if (fp.m == 7 && gen) TILE_LAUNCH(7, false, true, 512);
   else if (fp.m == 7) TILE_LAUNCH(7, false, false, 512);
