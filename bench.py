@@ -355,12 +355,12 @@ def main():
             log('no learned cost model (%s); greedy fusion' % e)
             method = 'greedy'
     t0 = time.perf_counter()
-    fused, blocks = fusion.fused_program(noisy, n, 4, method, cost, 'seq')
+    fused, blocks = fusion.fused_program(noisy, n, fusion.KMAX[method], method, cost, 'seq')
     plan_s = time.perf_counter() - t0
     plain = fusion.plain_program(noisy, n)
     ks = [len(q) if len(m) > 1 else 0 for q, m in blocks]
     log('plan: %s, %d passes for %d gates (%.2f s host); k histogram %s'
-        % (method, len(fused), G, plan_s, np.bincount(ks, minlength=5).tolist()))
+        % (method, len(fused), G, plan_s, np.bincount(ks, minlength=6).tolist()))
 
     log('timing plain fp64 ...')
     t_plain, f_plain = time_program(plain, n, 'fp64', args.steps, args.warmup)

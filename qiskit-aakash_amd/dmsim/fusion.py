@@ -23,6 +23,11 @@ from . import native, ptm
 from .program import Program
 
 NoisyGate = collections.namedtuple('NoisyGate', 'qubits R kind')
+
+# largest block each planner forms: the greedy rule stops at 4 qubits (a
+# 6-digit tile keeps runs of >= 4^2 coefficients); the cost model also
+# weighs 5-qubit blocks (7-digit tiles)
+KMAX = {'greedy': 4, 'learned': 5}
 # kind: 'u3' (4x4 R), 'cx' (R = per-pair scale16 or None, signed permutation),
 #       '2q' (16x16 R)
 
@@ -304,6 +309,6 @@ def plan_levels(levels, count, n, backend):
             if noisy_levels:
                 gates += [NoisyGate((q,), mem_R, 'u3') for q in range(n)]
             clock += 1
-        prog, _ = fused_program(gates, n, 4, method, cost)
+        prog, _ = fused_program(gates, n, KMAX[method], method, cost)
         out[start] = (clock, prog.run)
     return out

@@ -44,8 +44,8 @@ def ab_compare(model, n=16, seeds=(5000, 5001, 5002), gates=400, precision='fp32
     for s in seeds:
         noisy = fusion.noisy_gates(workloads.local_u3_cx(n, gates, s), 1e-3)
         model.precision = precision
-        learned, lb = fusion.fused_program(noisy, n, 5, 'learned', model)
-        greedy, gb = fusion.fused_program(noisy, n, 4, 'greedy', None, 'seq')
+        learned, lb = fusion.fused_program(noisy, n, fusion.KMAX['learned'], 'learned', model)
+        greedy, gb = fusion.fused_program(noisy, n, fusion.KMAX['greedy'], 'greedy', None, 'seq')
         plain = fusion.plain_program(noisy, n)
         row = {'seed': s, 'gates': gates, 'passes_learned': len(learned), 'passes_greedy': len(greedy),
                't_learned': time_program(learned, n, precision), 't_greedy': time_program(greedy, n, precision),

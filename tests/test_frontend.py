@@ -224,3 +224,14 @@ def test_fused_tile_choice():
     assert fusion.tile_for(n, [0, 1, 2]) == 6         # m=6 already gives w=3
     assert fusion.tile_for(n, [15, 14]) == 6          # block at the lowest positions
     assert fusion.tile_for(6, [0, 1, 2, 3]) == 6      # register too small for 7
+
+
+def test_shipped_cost_model_file():
+    """The trained parameters load without pickle and carry their training
+    report (held-out fit of the log-time target)."""
+    from dmsim import costmodel
+    m = costmodel.CostModel.load()
+    P = costmodel.N_FEATURES * costmodel.HIDDEN + costmodel.HIDDEN + costmodel.HIDDEN ** 2 + 2 * costmodel.HIDDEN + 1
+    assert m.params.shape == (P,) and np.isfinite(m.params).all()
+    assert m.meta['features'] == costmodel.N_FEATURES and m.meta['hidden'] == costmodel.HIDDEN
+    assert m.meta['holdout_r2'] > 0.9

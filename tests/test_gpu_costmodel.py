@@ -1,0 +1,87 @@
+"""The trained fusion cost model shipped with the package: its predictions on
+the GPU agree with the numpy forward pass, it ranks measured pass times, and
+its plans run faster than the greedy rule's on held-out circuits."""
+import time
+
+import numpy as np
+import pytest
+
+from dmsim import costmodel, fusion, native, workloads
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def _state(n, prec):
+    st = native.DeviceState(n, prec)
+    st.init_product(np.tile([1.0, 0.0, 0.0, 1.0], n), 0.5 ** n)
+    return st
+
+
+def _seconds(prog, st, reps=2):
+    prog.run(st)
+    st.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        prog.run(st)
+    st.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def test_shipped_model_predicts_like_the_oracle():
+    model = costmodel.default_model()
+    rows = [('seq', [0, 1, 2, 3], 5, 3), ('dense', [4, 5, 6], 4, 2), ('plain1q', [7], 1, 0), ('plaincx', [2, 3], 0, 1)]
+    X = np.stack([costmodel.features(k, 16, q, a, b, 'fp32') for k, q, a, b in rows])
+    got = native.mlp_predict(model.params, costmodel.N_FEATURES, costmodel.HIDDEN, X)
+    want, _ = mlp.forward(model.params.astype(np.float64), costmodel.N_FEATURES, costmodel.HIDDEN,
+                          X.astype(np.float64))
+    assert np.max(np.abs(got - want)) <= 1e-4 * (1 + np.max(np.abs(want)))
+
+
+def test_model_ranks_measured_pass_times():
+    """Spearman rank correlation >= 0.8 between predicted and measured times
+    of passes drawn from held-out circuits (seeds outside the training set)."""
+    n = 16
+    model = costmodel.default_model()
+    specs = costmodel.candidate_passes(range(7000, 7003), n)
+    rng = np.random.default_rng(0)
+    pick = [specs[i] for i in rng.choice(len(specs), 24, replace=False)]
+    st = _state(n, 'fp32')
+    measured = [costmodel.time_pass(st, kind, qs, gs, reps=2) for kind, qs, gs in pick]
+    st.close()
+    rows = []
+    for kind, qs, gs in pick:
+        n2 = sum(1 for g in gs if len(g.qubits) == 2)
+        rows.append((kind, qs, len(gs) - n2, n2))
+    pred = model.predict_many(rows, n, 'fp32')
+    rank = lambda v: np.argsort(np.argsort(v))
+    rho = np.corrcoef(rank(np.asarray(measured)), rank(pred))[0, 1]
+    assert rho >= 0.8, (rho, measured, pred.tolist())
+
+
+def test_learned_plan_beats_greedy_on_held_out_circuits():
+    """Measured time of the learned plan vs the greedy k <= 4 plan, FP32,
+    16 qubits, two held-out 300-gate circuits; results agree to FP32 accuracy."""
+    n = 16
+    model = costmodel.default_model()
+    model.precision = 'fp32'
+    t_l = t_g = 0.0
+    for seed in (7100, 7101):
+        noisy = fusion.noisy_gates(workloads.local_u3_cx(n, 300, seed), 1e-3)
+        learned, _ = fusion.fused_program(noisy, n, fusion.KMAX['learned'], 'learned', model)
+        greedy, _ = fusion.fused_program(noisy, n, fusion.KMAX['greedy'], 'greedy', None, 'seq')
+        a = _state(n, 'fp32')
+        b = _state(n, 'fp32')
+        t_l += _seconds(learned, a)
+        t_g += _seconds(greedy, b)
+        a.close()
+        b.close()
+        a = _state(n, 'fp32')
+        b = _state(n, 'fp32')
+        learned.run(a)
+        greedy.run(b)
+        cmp = a.compare(b)
+        a.close()
+        b.close()
+        assert cmp['max_abs_diff'] <= 1e-6 * 2.0 ** -n * 50
+    assert t_l < t_g, (t_l, t_g)
