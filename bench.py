@@ -21,6 +21,7 @@ import argparse
 import json
 import math
 import os
+import re
 import sys
 import time
 
@@ -35,6 +36,10 @@ import numpy as np  # noqa: E402
 from dmsim import fusion, native, workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+# over `bench.py --kernels`, each divided by the counter's reading on a pass
+# of known traffic (the diagonal memory-noise pass)
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'headline_16q_hbm_traffic.md')
 MFMA_PEAK_TFLOPS = {'fp32': 157.3, 'fp64': 78.6}
 METRIC = 'noisy gates/sec at 16q: learned-fused FP32 vs plain FP64; % HBM peak'
 
@@ -99,6 +104,19 @@ def kernel_roofline(fams, n, precision):
     ach = byts / avg_s / 1e9
     return dom, {'bound': 'hbm', 'achieved': ach, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                  'frac': ach / HBM_PEAK_GBS, 'traffic': None}, avg_s, byts, 0.0
+
+
+def measured_traffic(key, plan):
+    """Recorded HBM bytes per launch of kernel family ``key`` under ``plan``
+    ('learned/104', or 'any' for plan-independent kernels), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            text = f.read()
+    except OSError:
+        return None
+    m = re.search(r'^traffic_bytes_per_launch %s %s: ([0-9.eE+]+)\s*$' % (re.escape(key), re.escape(plan)), text,
+                  re.M)
+    return float(m.group(1)) if m else None
 
 
 def streaming_fraction(fams, n, precision, counts):
@@ -175,38 +193,56 @@ def cpu_baseline(n_target, G, seed, budget_s=20.0, n_sample=13):
                       '%.3f gates/s; scaled by 4^-%d to %d qubits' % (done, n, el, rate, n_target - n, n_target)}
 
 
-def kernels_mode(n=16, reps=3):
-    """One short run of each kernel family at n qubits (for rocprofv3 counter
-    passes): a diagonal memory-noise pass (reads and writes every coefficient
-    once: the traffic calibration), plain 1q / CX passes and fused passes."""
+def headline_plan(noisy, n, fusion_arg='auto'):
+    """(program, blocks, method, plan seconds) of the headline fused path:
+    the learned planner when the trained model ships, else greedy."""
+    method = fusion_arg
+    cost = None
+    if method in ('auto', 'learned'):
+        try:
+            from dmsim import costmodel
+            cost = costmodel.default_model()
+            cost.precision = 'fp32'
+            method = 'learned'
+        except Exception as e:  # no trained model shipped
+            if fusion_arg == 'learned':
+                raise
+            log('no learned cost model (%s); greedy fusion' % e)
+            method = 'greedy'
+    t0 = time.perf_counter()
+    prog, blocks = fusion.fused_program(noisy, n, fusion.KMAX[method], method, cost, 'seq')
+    return prog, blocks, method, time.perf_counter() - t0
+
+
+def kernels_mode(args):
+    """One short run for rocprofv3 counter passes, in this order:
+      1. calibration: a diagonal memory-noise pass (k_diag_pow), which reads
+         and writes every coefficient exactly once (N*s bytes each way), FP64
+         then FP32, 3 launches each;
+      2. one application of the headline fused FP32 program (same plan);
+      3. the first 60 gates of the plain FP64 program."""
     from dmsim.program import Program
-    out = {}
-    noisy = fusion.noisy_gates(workloads.local_u3_cx(n, 200, 0), 1e-3)
-    fused, blocks = fusion.fused_program(noisy, n, 4, 'greedy', None, 'seq')
-    big = max(blocks, key=lambda b: len(b[1]))
+    n, G = args.qubits, args.gates
+    noisy = fusion.noisy_gates(workloads.local_u3_cx(n, G, args.seed), 1e-3)
+    fused, blocks, method, _ = headline_plan(noisy, n, args.fusion)
+    out = {'n': n, 'plan': '%s/%d' % (method, len(fused)), 'calib_launches': 3}
     for prec in ('fp64', 'fp32'):
         st = native.DeviceState(n, prec)
         zero_state(st, n)
-        progs = {
-            'calib_diag': (lambda p: p.memory_noise(0.99, 0.5, 0.98)),
-            'plain_1q_high': (lambda p: p.one_qubit(0, noisy[0].R if noisy[0].kind == 'u3' else np.eye(4))),
-            'plain_1q_low': (lambda p: p.one_qubit(n - 1, np.eye(4))),
-            'plain_cx': (lambda p: p.cx(3, 4, None)),
-            'fused_block': (lambda p: fusion.add_block(p, noisy, big[0], big[1], 'seq')),
-        }
-        for name, fill in progs.items():
-            prog = Program(n)
-            for _ in range(reps):
-                fill(prog)
-            prog.run(st)
+        prog = Program(n)
+        for _ in range(3):
+            prog.memory_noise(0.99, 0.5, 0.98)   # thermal feed (1-g)(2p-1) = 0: one diagonal pass
+        prog.run(st)
+        st.synchronize()
+        if prec == 'fp32':
+            fused.run(st)
             st.synchronize()
-            st.profile(True)
-            st.profile_reset()
-            prog.run(st)
-            tot = sum(st.profile_read(f)[1] for f in native.FAMILIES)
-            st.profile(False)
-            out['%s_%s_ms' % (name, prec)] = tot / reps
         st.close()
+    st = native.DeviceState(n, 'fp64')
+    zero_state(st, n)
+    fusion.plain_program(noisy[:60], n).run(st)
+    st.synchronize()
+    st.close()
     return out
 
 
@@ -331,7 +367,7 @@ def main():
         sys.exit(2)
 
     if args.kernels:
-        print(json.dumps(kernels_mode(args.qubits)), flush=True)
+        print(json.dumps(kernels_mode(args)), flush=True)
         return
     if args.config != 'headline':
         fn = {'batch': batch_mode, 'precision': precision_mode, 'thermal': thermal_mode}[args.config]
@@ -342,21 +378,7 @@ def main():
     gates = workloads.local_u3_cx(n, G, args.seed)
     noisy = fusion.noisy_gates(gates, 1e-3)
 
-    method = args.fusion
-    cost = None
-    if method in ('auto', 'learned'):
-        try:
-            from dmsim import costmodel
-            cost = costmodel.default_model()
-            method = 'learned'
-        except Exception as e:  # no trained model shipped
-            if args.fusion == 'learned':
-                raise
-            log('no learned cost model (%s); greedy fusion' % e)
-            method = 'greedy'
-    t0 = time.perf_counter()
-    fused, blocks = fusion.fused_program(noisy, n, fusion.KMAX[method], method, cost, 'seq')
-    plan_s = time.perf_counter() - t0
+    fused, blocks, method, plan_s = headline_plan(noisy, n, args.fusion)
     plain = fusion.plain_program(noisy, n)
     ks = [len(q) if len(m) > 1 else 0 for q, m in blocks]
     log('plan: %s, %d passes for %d gates (%.2f s host); k histogram %s'
@@ -385,6 +407,9 @@ def main():
     value = gps(t_fused)
     dom, roof, avg_s, byts, flops = kernel_roofline(f_fused, n, 'fp32')
     pdom, proof, _, _, _ = kernel_roofline(f_plain, n, 'fp64')
+    if n == 16 and G == 1000 and args.seed == 0:
+        roof['traffic'] = measured_traffic('%s_fp32' % dom, '%s/%d' % (method, len(fused)))
+        proof['traffic'] = measured_traffic('plain_fp64_%s' % pdom, 'any')
     acc = None if args.no_accuracy else accuracy(fused, plain, n)
     if acc:
         log('accuracy: %s' % json.dumps(acc))
