@@ -248,7 +248,14 @@ def kernels_mode(args):
 
 def batch_mode(args):
     from dmsim import batch
-    specs = workloads.batch(512, args.seed)
+    if args.batch_circuits > 64:
+        # 512 circuits of 6-12 qubits enqueue ~4e5 kernel launches per run; a
+        # run of that batch ended the process on MI355X without an error
+        # message, and it stays off until the cause is known
+        print('batch config: %d circuits refused (at most 64 run on this build)' % args.batch_circuits,
+              file=sys.stderr)
+        sys.exit(3)
+    specs = workloads.batch(args.batch_circuits, args.seed)
     lines = {}
     for prec in ('fp64', 'fp32'):
         (lmax, thr), tune = batch.autotune(specs, prec)
@@ -284,6 +291,7 @@ def precision_mode(args):
     prog, blocks = fusion.fused_program(noisy, n, 4, 'greedy', None, 'seq')
     res = {}
     states = {}
+    log('precision config: %d gates, %d passes' % (len(noisy), len(prog)))
     for prec in ('fp64', 'fp32'):
         st = native.DeviceState(n, prec)
         zero_state(st, n)
@@ -292,6 +300,7 @@ def precision_mode(args):
         prog.run(st)
         st.synchronize()
         res[prec + '_gates_per_s'] = len(noisy) / (time.perf_counter() - t0)
+        log('  %s: %.1f gates/s' % (prec, res[prec + '_gates_per_s']))
         states[prec] = st
     cmp = states['fp32'].compare(states['fp64'])
     fid = cmp['dot'] / math.sqrt(cmp['norm2_self'] * cmp['norm2_other'])
@@ -358,8 +367,9 @@ def main():
     ap.add_argument('--fusion', default='auto', choices=['auto', 'greedy', 'learned'])
     ap.add_argument('--quick', action='store_true', help='time only the plain fp64 and headline paths')
     ap.add_argument('--config', default='headline', choices=['headline', 'batch', 'precision', 'thermal'])
-    ap.add_argument('--layers', type=int, default=40, help='precision config: layers')
+    ap.add_argument('--layers', type=int, default=400, help='precision config: layers')
     ap.add_argument('--kernels', action='store_true', help='short per-kernel run (for rocprofv3 counters)')
+    ap.add_argument('--batch-circuits', type=int, default=512, help='batch config: circuits')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU: the path has no multi-GPU form in this build (--gpus %d refused)'

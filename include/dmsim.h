@@ -201,7 +201,9 @@ int dm_mlp_train(float *params, int n_features, int hidden, const float *X, cons
  * workgroup's LDS (one launch per width, lds_threads threads per workgroup);
  * the others keep their state in HBM and run on 8 parallel streams.
  * dm_batch_run with use_graph != 0 captures the whole batch into one hipGraph
- * on first use and replays it; per-circuit times come from GPU clock stamps. */
+ * on first use and replays it (batches of at most 16384 kernel launches per
+ * run; larger ones return DM_ERR_ARG); per-circuit times come from GPU clock
+ * stamps. */
 typedef struct dm_batch dm_batch;
 int dm_batch_create(int dtype, int ncirc, const int *n_qubits, const int64_t *op_start, const dm_op *ops,
                     const double *params, int64_t nparams, const double *meas_err, int lds_max_qubits,

@@ -2007,7 +2007,15 @@ struct dm_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double clock_khz = 1.0e5;
+  int64_t launches = 0;  // kernel launches one run enqueues
 };
+
+// Largest batch (in kernel launches per run) that dm_batch_run captures into
+// a hipGraph.  A 512-circuit batch of 6-12 qubit circuits with per-gate
+// memory noise enqueues ~4e5 launches; such a run ended the process on
+// MI355X without an error message, and graph capture at that size stays off
+// until the cause is known.
+constexpr int64_t kBatchGraphLaunchLimit = 16384;
 
 namespace {
 
@@ -2087,6 +2095,21 @@ int small_circuit_ops(int n, const dm_op *ops, int nops, const double *params, i
 
 size_t lds_bytes(int n, int dtype) {
   return ((size_t)1 << (2 * n)) * (dtype == DM_FP64 ? 8 : 4) + ((size_t)1 << n) * 8;
+}
+
+// Kernel launches the HBM-resident part of one circuit enqueues per run.
+int64_t circuit_launches(const dm_state *h, const Prepared &pp) {
+  int64_t k = 2 + 1 + 1 + h->n;  // clock stamps, |0..0> init, gather + n contractions
+  for (const dm_op &op : pp.ops) {
+    if (op.kind != DM_OP_MEMORY_NOISE) {
+      k += 1;
+      continue;
+    }
+    const double *pv = pp.params.data() + op.param;
+    if (pv[0] == 1.0 && pv[2] == 1.0) continue;
+    k += ((1 - pv[2]) * (2 * pv[1] - 1) == 0.0) ? 1 : 1 + (h->n - 1 + 2) / 3;
+  }
+  return k;
 }
 
 template <typename T>
@@ -2216,6 +2239,7 @@ static int batch_create_t(dm_batch *b, const int64_t *op_start, const dm_op *ops
     }
   }
   b->small_width_start[8] = (int)b->small.size();
+  for (int w = 1; w <= 7; ++w) b->launches += b->small_width_start[w + 1] > b->small_width_start[w] ? 1 : 0;
   if (!circs.empty()) {
     std::vector<T> tv(vals.begin(), vals.end());
     HIP_TRY(hipMalloc(&b->d_circs, circs.size() * sizeof(BatchCirc)));
@@ -2239,6 +2263,7 @@ static int batch_create_t(dm_batch *b, const int64_t *op_start, const dm_op *ops
     Prepared pp;
     if (int rc = prepare_program<T>(h, ops + op_start[c], (int)(op_start[c + 1] - op_start[c]), params, nparams, &pp))
       return rc;
+    b->launches += circuit_launches(h, pp);
     b->progs.push_back(std::move(pp));
     double *sc = nullptr;
     HIP_TRY(hipMalloc(&sc, 2 * ((size_t)1 << h->n) * sizeof(double)));
@@ -2620,6 +2645,9 @@ int dm_batch_create(int dtype, int ncirc, const int *n_qubits, const int64_t *op
 
 int dm_batch_run(dm_batch *b, int use_graph, double *probs_out, double *circuit_us, double *total_ms) {
   if (!b) return fail(DM_ERR_ARG, "null batch");
+  if (use_graph && b->launches > kBatchGraphLaunchLimit)
+    return fail(DM_ERR_ARG, "batch run of %lld kernel launches exceeds the graph capture limit (%lld)",
+                (long long)b->launches, (long long)kBatchGraphLaunchLimit);
   if (use_graph && !b->exec) {
     HIP_TRY(hipStreamBeginCapture(b->master, hipStreamCaptureModeThreadLocal));
     const int rc = b->dtype == DM_FP64 ? enqueue_batch<double>(b) : enqueue_batch<float>(b);
