@@ -14,9 +14,9 @@ from the pass's features:
 The target is ln(time / state bytes), so one network covers register sizes.
 Training data are GPU timings of passes drawn from the candidate blocks of
 seeded random circuits (``collect``).  The fusion planner (dmsim.fusion,
-method 'learned') asks ``block_time`` for each candidate block and keeps the
-one with the most gates per predicted second; ``best_mode`` picks the
-cheaper tile form.
+method 'learned') asks ``block_times`` for the candidate blocks of each step
+and keeps the one with the most gates per predicted second; ``best_mode``
+picks the cheaper tile form.
 """
 import json
 import math
@@ -125,6 +125,30 @@ class CostModel:
         td = self._time('dense', n, qs, n1, n2) if len(qs) <= 4 else float('inf')
         return 'seq' if ts <= td else 'dense'
 
+    def block_times(self, n, blocks):
+        """Predicted seconds of each (qubits, gates) block (one batched
+        prediction for the rows not cached yet)."""
+        rows, keys = [], []
+        for qs, gates in blocks:
+            n1, n2 = self._counts(gates)
+            kind = pass_kind(qs, gates, 'seq')
+            want = [kind] if kind != 'seq' else (['seq', 'dense'] if 2 <= len(qs) <= 4 else ['seq'])
+            ks = []
+            for kd in want:
+                key = (kd, n, tuple(sorted(qs)), n1, n2, self.precision)
+                ks.append(key)
+                if key not in self._cache:
+                    rows.append((kd, qs, n1, n2))
+            keys.append(ks)
+        if rows:
+            uniq = {}
+            for r in rows:
+                uniq.setdefault((r[0], n, tuple(sorted(r[1])), r[2], r[3], self.precision), r)
+            t = self.predict_many(list(uniq.values()), n, self.precision)
+            for key, tv in zip(uniq.keys(), t):
+                self._cache[key] = float(tv)
+        return [min(self._cache[k] for k in ks) for ks in keys]
+
     def block_time(self, n, qs, gates):
         n1, n2 = self._counts(gates)
         kind = pass_kind(qs, gates, 'seq')
@@ -153,9 +177,10 @@ def default_model():
 # data collection and training (run on the GPU)
 # ---------------------------------------------------------------------------
 
-def candidate_passes(seeds, n, gates_per_circuit=200, kmax=5):
+def candidate_passes(seeds, n, gates_per_circuit=200, kmax=4):
     """Distinct pass specs (kind, qubits, gate list) drawn from the candidate
-    blocks the planners consider on seeded random circuits."""
+    blocks the planners consider (fusion.candidate_blocks) on seeded random
+    circuits; each circuit advances by a randomly chosen candidate."""
     specs = {}
     for seed in seeds:
         noisy = fusion.noisy_gates(workloads.local_u3_cx(n, gates_per_circuit, seed), 1e-3)
@@ -167,9 +192,7 @@ def candidate_passes(seeds, n, gates_per_circuit=200, kmax=5):
                 first += 1
             if first >= len(noisy):
                 break
-            options = [(list(noisy[first].qubits), [first])]
-            for k in range(2, kmax + 1):
-                options.append(fusion._grow(noisy, done, first, k, n))
+            options = fusion.candidate_blocks(noisy, done, first, kmax, n)
             for qs, mem in options:
                 gs = [noisy[i] for i in mem]
                 n2 = sum(1 for g in gs if len(g.qubits) == 2)

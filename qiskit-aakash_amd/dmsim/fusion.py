@@ -9,13 +9,18 @@ larger block is one tile pass (DM_OP_FUSED): the gates applied in sequence to
 LDS-staged tiles, or the block's dense PTM on the matrix cores.  ``plan`` chooses the blocks:
 
 * ``greedy``  -- grow each block up to k_max qubits (the classic rule);
-* ``learned`` -- at every step try k = 1..4 and take the block with the most
-  gates per predicted microsecond, the prediction coming from the MLP cost
-  model (dmsim.costmodel); a single gate on its plain kernel is always a
-  candidate, so the plan can never be worse than per-gate application under
-  the model.
+* ``learned`` -- at every step score the candidate blocks seeded at the first
+  pending gate (the greedy growth for every k, and every qubit set of at most
+  k_max qubits drawn from the nearby two-qubit gates, ``candidate_blocks``)
+  and take the one with the largest predicted net saving -- its gates valued
+  at the plan's running cost per gate, minus TIME_WEIGHT times its predicted
+  time -- the predictions coming from the MLP cost model (dmsim.costmodel); a
+  single gate on its plain kernel is always a candidate;
+* ``search``  -- the same candidates, scored by gate count alone (every pass
+  costed alike): the learned planner without its network.
 """
 import collections
+import itertools
 
 import numpy as np
 
@@ -24,10 +29,15 @@ from .program import Program
 
 NoisyGate = collections.namedtuple('NoisyGate', 'qubits R kind')
 
-# largest block each planner forms: the greedy rule stops at 4 qubits (a
-# 6-digit tile keeps runs of >= 4^2 coefficients); the cost model also
-# weighs 5-qubit blocks (7-digit tiles)
-KMAX = {'greedy': 4, 'learned': 5}
+# largest block each planner forms (qubits): 4, a Pauli transfer matrix of at
+# most 256 x 256
+KMAX = {'greedy': 4, 'learned': 4, 'search': 4}
+
+# weight of a block's predicted time against the gates it takes, both in
+# seconds (gates at the plan's running cost per gate).  Measured on MI355X
+# (16 qubits, FP32, five held-out circuits): weights 1.0 and 0.5 plan 3 % and
+# 1.5 % slower than 0.25 and 0.1, which match the gate-count search.
+TIME_WEIGHT = 0.25
 # kind: 'u3' (4x4 R), 'cx' (R = per-pair scale16 or None, signed permutation),
 #       '2q' (16x16 R)
 
@@ -116,6 +126,77 @@ def _grow(gates, done, start, kmax, n):
     return qs, members
 
 
+def _absorb(gates, done, start, qset):
+    """Gates of the block on qubit set ``qset`` seeded at the first pending
+    gate ``start``: scanning forward, a gate joins when its qubits lie in the
+    set and none is blocked; any other gate blocks all of its qubits."""
+    S = set(qset)
+    members = []
+    blocked = set()
+    for i in range(start, len(gates)):
+        if done[i]:
+            continue
+        gq = gates[i].qubits
+        if any(q in blocked for q in gq) or not S.issuperset(gq):
+            blocked.update(gq)
+            if S <= blocked:
+                break
+        else:
+            members.append(i)
+    return members
+
+
+def _near_qubits(gates, done, start, seed, limit=6, window=48):
+    """Up to ``limit`` other qubits, in order of first appearance, of the
+    pending two-qubit gates within ``window`` gates that connect to the seed
+    qubits (directly or through qubits already collected)."""
+    reach = set(seed)
+    out = []
+    seen = 0
+    for i in range(start, len(gates)):
+        if done[i]:
+            continue
+        seen += 1
+        if seen > window or len(out) >= limit:
+            break
+        gq = gates[i].qubits
+        if len(gq) == 2 and (gq[0] in reach or gq[1] in reach):
+            for q in gq:
+                if q not in reach:
+                    reach.add(q)
+                    out.append(q)
+    return out[:limit]
+
+
+def candidate_blocks(gates, done, start, kmax, n):
+    """Distinct candidate blocks (qubits, members) seeded at ``start``: the
+    greedy growth for every k <= kmax, and every seed-containing set of at most
+    kmax qubits drawn from the nearby qubits (its members by ``_absorb``; its
+    qubits cut down to those its members use)."""
+    seed = list(gates[start].qubits)
+    cands = {}
+
+    def add(members):
+        key = tuple(members)
+        if key in cands or not members:
+            return
+        qs = []
+        for i in members:
+            for q in gates[i].qubits:
+                if q not in qs:
+                    qs.append(q)
+        cands[key] = (qs, members)
+
+    add([start])
+    for k in range(2, kmax + 1):
+        add(_grow(gates, done, start, k, n)[1])
+    near = _near_qubits(gates, done, start, seed)
+    for extra in range(1, kmax - len(seed) + 1):
+        for T in itertools.combinations(near, extra):
+            add(_absorb(gates, done, start, seed + list(T)))
+    return list(cands.values())
+
+
 def min_block_qubits(n):
     """The MFMA block kernel needs n >= k + column qubits (3+3 or 4+2)."""
     return 6
@@ -127,6 +208,7 @@ def plan(gates, n, kmax=4, method='greedy', cost=None):
     blocks = []
     first = 0
     allow_mfma = n >= min_block_qubits(n)
+    spent = [0.0, 0]  # predicted seconds and gates of the blocks chosen so far
     while True:
         while first < len(gates) and done[first]:
             first += 1
@@ -135,25 +217,26 @@ def plan(gates, n, kmax=4, method='greedy', cost=None):
         if method == 'greedy':
             km = kmax if allow_mfma else min(kmax, 2)
             qs, mem = _grow(gates, done, first, km, n)
-        elif method == 'learned':
-            best = None
-            seen = set()
-            for k in range(1, kmax + 1):
-                if k >= 3 and not allow_mfma:
-                    break
-                if k == 1:
-                    qs_k, mem_k = list(gates[first].qubits), [first]
+        elif method in ('learned', 'search'):
+            km = kmax if allow_mfma else min(kmax, 2)
+            cands = candidate_blocks(gates, done, first, km, n)
+            if method == 'learned':
+                # predicted net saving: the gates a block takes, valued at the
+                # plan's running cost per gate, minus its weighted own time
+                t = cost.block_times(n, [(qs_c, [gates[i] for i in mem_c]) for qs_c, mem_c in cands])
+                g = [len(c[1]) for c in cands]
+                if spent[1]:
+                    per_gate = spent[0] / spent[1]
                 else:
-                    qs_k, mem_k = _grow(gates, done, first, k, n)
-                key = tuple(mem_k)
-                if key in seen:
-                    continue
-                seen.add(key)
-                t = cost.block_time(n, qs_k, [gates[i] for i in mem_k])
-                score = len(mem_k) / t
-                if best is None or score > best[2]:
-                    best = (qs_k, mem_k, score)
-            qs, mem = best[0], best[1]
+                    j = int(np.argmax(g))
+                    per_gate = t[j] / g[j]
+                score = [gc * per_gate - TIME_WEIGHT * tc for gc, tc in zip(g, t)]
+                spent[0] += t[int(np.argmax(score))]
+                spent[1] += g[int(np.argmax(score))]
+            else:
+                # the same search with every pass costed alike: most gates
+                score = [len(c[1]) for c in cands]
+            qs, mem = cands[int(np.argmax(score))]
         else:
             raise ValueError('unknown fusion method %r' % method)
         for i in mem:

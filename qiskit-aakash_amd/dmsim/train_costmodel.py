@@ -6,7 +6,9 @@
    local U3/CX generator), times a sample of them on the GPU in FP32 and FP64;
 2. trains the perceptron on the GPU (dmsim.costmodel.train) and saves it;
 3. A/B: on held-out circuits (seeds 5000+) times the learned plan, the greedy
-   plan (k <= 4, sequential tile passes) and plain per-gate application.
+   plan (k <= 4, sequential tile passes), the same candidate search scored by
+   gate count alone (the learned planner without its network) and plain
+   per-gate application.
 Writes a JSON report next to the parameters.
 """
 import argparse
@@ -46,11 +48,14 @@ def ab_compare(model, n=16, seeds=(5000, 5001, 5002), gates=400, precision='fp32
         model.precision = precision
         learned, lb = fusion.fused_program(noisy, n, fusion.KMAX['learned'], 'learned', model)
         greedy, gb = fusion.fused_program(noisy, n, fusion.KMAX['greedy'], 'greedy', None, 'seq')
+        search, sb = fusion.fused_program(noisy, n, fusion.KMAX['search'], 'search', None, 'seq')
         plain = fusion.plain_program(noisy, n)
         row = {'seed': s, 'gates': gates, 'passes_learned': len(learned), 'passes_greedy': len(greedy),
+               'passes_search': len(search),
                't_learned': time_program(learned, n, precision), 't_greedy': time_program(greedy, n, precision),
-               't_plain': time_program(plain, n, precision, reps=1)}
+               't_search': time_program(search, n, precision), 't_plain': time_program(plain, n, precision, reps=1)}
         row['gain_vs_greedy'] = row['t_greedy'] / row['t_learned']
+        row['gain_vs_search'] = row['t_search'] / row['t_learned']
         row['gain_vs_plain'] = row['t_plain'] / row['t_learned']
         log('A/B seed %d: %s' % (s, json.dumps(row)))
         out.append(row)

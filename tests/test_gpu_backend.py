@@ -43,10 +43,10 @@ def fresh_backend_state():
     reset()
 
 
-def run_both(qc, opts, precision='fp64'):
+def run_both(qc, opts, precision='fp64', **backend_only):
     exp = assemble(qc).experiments[0]
     ref, count, _ = refsim.run_experiment(copy.deepcopy(exp.instructions), qc.num_qubits, dict(opts))
-    res = execute(qc, backend(), precision=precision, **opts).result()
+    res = execute(qc, backend(), precision=precision, **opts, **backend_only).result()
     out = res['results'][0]
     assert out['number_of_clock_cycles'] == count
     return ref, out['data'], res
@@ -118,6 +118,22 @@ def test_random_circuits_all_options_fp64(seed):
                                                               'rz': [0.995, -0.02]},
             'tsp_model_error': [0.98, 0.03], 'gate_depolarization': 0.002}
     ref, got, _ = run_both(qc, opts)
+    compare(ref, got)
+
+
+@pytest.mark.parametrize('fusion', ['greedy', 'learned'])
+@pytest.mark.parametrize('seed,barriers', [(11, False), (12, True)])
+def test_fused_execution_matches_oracle_fp64(fusion, seed, barriers):
+    """Fused execution (tile passes, memory noise as per-qubit channels)
+    against the oracle's level-by-level run: every noise option, TSP CX
+    (dense two-qubit operations), with and without barriers."""
+    n = 8
+    gates = workloads.local_u3_cx(n, 120, seed)
+    qc = workloads.to_circuit(n, gates, barrier_after_each=barriers, measure=('ensemble', 'Z'))
+    opts = {'decay_factor': 0.98, 'decoherence_factor': 0.97, 'thermal_factor': 0.3,
+            'rotation_error': {'rx': [1.0, 0.0], 'ry': [0.99, 0.01], 'rz': [0.995, -0.02]},
+            'tsp_model_error': [0.98, 0.03], 'gate_depolarization': 0.002}
+    ref, got, _ = run_both(qc, opts, fusion=fusion)
     compare(ref, got)
 
 

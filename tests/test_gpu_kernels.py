@@ -308,6 +308,29 @@ def test_fused_tile_sizes_match_plain(prec, tile):
     assert np.max(np.abs(got - want)) <= t
 
 
+@pytest.mark.parametrize('prec', PRECS)
+def test_fused_passes_with_dense_two_qubit_ops(prec):
+    """TSP CX gates are dense 16x16 operations inside the tile passes."""
+    from dmsim import fusion, workloads
+    n = 8
+    gates = fusion.noisy_gates(workloads.local_u3_cx(n, 150, 3), 1e-3, tsp=(0.97, 0.05))
+    assert any(g.kind == '2q' for g in gates)
+    rng = np.random.default_rng(8)
+    a = rand_state(n, rng)
+    plain = fusion.plain_program(gates, n)
+    fused, _ = fusion.fused_program(gates, n, 4, 'greedy', None, 'seq')
+    s1 = make(n, 'fp64', a)
+    plain.run(s1)
+    want = s1.download()
+    s1.close()
+    s2 = make(n, prec, a)
+    fused.run(s2)
+    got = s2.download()
+    s2.close()
+    t = 1e-14 * 2.0 ** -n * 50 if prec == 'fp64' else 2e-6 * 2.0 ** -n * 50
+    assert np.max(np.abs(got - want)) <= t
+
+
 def test_fused_pass_rejects_bad_records():
     from dmsim.program import Program
     st = native.DeviceState(6, 'fp64')
