@@ -60,6 +60,33 @@ template <typename P, typename T>
 __device__ __forceinline__ void stp(T *p, const P &x) {
   *reinterpret_cast<P *>(p) = x;
 }
+
+// 16-byte loads / stores, with the nontemporal hint when NT (streamed data
+// not re-read before eviction; measured faster on gfx950 for strides of
+// >= 16 coefficients, slower for stride 4).
+template <typename T, bool NT>
+__device__ __forceinline__ Pack<T, 16 / sizeof(T)> ld16(const T *p) {
+  using P = Pack<T, 16 / sizeof(T)>;
+  if (!NT) return ldp<P>(p);
+  using V = T __attribute__((ext_vector_type(16 / sizeof(T))));
+  const V v = __builtin_nontemporal_load(reinterpret_cast<const V *>(p));
+  P r;
+#pragma unroll
+  for (int e = 0; e < (int)(16 / sizeof(T)); ++e) r.v[e] = v[e];
+  return r;
+}
+template <typename T, bool NT>
+__device__ __forceinline__ void st16(T *p, const Pack<T, 16 / sizeof(T)> &x) {
+  if (!NT) {
+    stp(p, x);
+    return;
+  }
+  using V = T __attribute__((ext_vector_type(16 / sizeof(T))));
+  V v;
+#pragma unroll
+  for (int e = 0; e < (int)(16 / sizeof(T)); ++e) v[e] = x.v[e];
+  __builtin_nontemporal_store(v, reinterpret_cast<V *>(p));
+}
 // Insert a zero 2-bit digit at each of the ascending positions pos[0..m-1].
 __device__ __forceinline__ uint64_t insert_digits(uint64_t r, const int *pos, int m) {
   for (int j = 0; j < m; ++j) {
@@ -91,7 +118,7 @@ Rows3<T> make_rows3(const double *R) {
 
 // Stride S = 4^p >= W: each unit covers W consecutive low indices of one
 // (hi) slab; the four components are four coalesced W-vectors.
-template <typename T, bool USE_I>
+template <typename T, bool USE_I, bool NT>
 __global__ __launch_bounds__(kBlock) void k_1q_wide(T *__restrict__ a, uint64_t units, int sh,
                                                     uint64_t S, Rows3<T> m) {
   constexpr int W = 16 / sizeof(T);
@@ -101,8 +128,8 @@ __global__ __launch_bounds__(kBlock) void k_1q_wide(T *__restrict__ a, uint64_t 
   for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < units; u += step) {
     T *b = a + ((u >> sh) * 4ull) * S + (u & mask) * W;
     P x0;
-    if (USE_I) x0 = ldp<P>(b);
-    const P x1 = ldp<P>(b + S), x2 = ldp<P>(b + 2 * S), x3 = ldp<P>(b + 3 * S);
+    if (USE_I) x0 = ld16<T, NT>(b);
+    const P x1 = ld16<T, NT>(b + S), x2 = ld16<T, NT>(b + 2 * S), x3 = ld16<T, NT>(b + 3 * S);
     P y1, y2, y3;
 #pragma unroll
     for (int e = 0; e < W; ++e) {
@@ -118,9 +145,9 @@ __global__ __launch_bounds__(kBlock) void k_1q_wide(T *__restrict__ a, uint64_t 
       y2.v[e] = s2;
       y3.v[e] = s3;
     }
-    stp(b + S, y1);
-    stp(b + 2 * S, y2);
-    stp(b + 3 * S, y3);
+    st16<T, NT>(b + S, y1);
+    st16<T, NT>(b + 2 * S, y2);
+    st16<T, NT>(b + 3 * S, y3);
   }
 }
 
@@ -192,7 +219,7 @@ __host__ __device__ constexpr bool cx_writes(bool ctrl_hi, bool scaled, int ih, 
 }
 
 // Low position >= 1 (stride >= W): units of W consecutive low indices.
-template <typename T, bool CTRL_HI, bool SCALED>
+template <typename T, bool CTRL_HI, bool SCALED, bool NT>
 __global__ __launch_bounds__(kBlock) void k_cx_wide(T *__restrict__ a, uint64_t units, int sh_lo,
                                                     int sh_mid, uint64_t Sh, uint64_t Sl,
                                                     Scale16<T> sc) {
@@ -210,7 +237,7 @@ __global__ __launch_bounds__(kBlock) void k_cx_wide(T *__restrict__ a, uint64_t 
     for (int ih = 0; ih < 4; ++ih)
 #pragma unroll
       for (int il = 0; il < 4; ++il)
-        if (cx_writes(CTRL_HI, SCALED, ih, il)) x[ih][il] = ldp<P>(b + ih * Sh + il * Sl);
+        if (cx_writes(CTRL_HI, SCALED, ih, il)) x[ih][il] = ld16<T, NT>(b + ih * Sh + il * Sl);
     // per-lane permutation
     P y[4][4];
 #pragma unroll
@@ -230,7 +257,7 @@ __global__ __launch_bounds__(kBlock) void k_cx_wide(T *__restrict__ a, uint64_t 
     for (int ih = 0; ih < 4; ++ih)
 #pragma unroll
       for (int il = 0; il < 4; ++il)
-        if (cx_writes(CTRL_HI, SCALED, ih, il)) stp(b + ih * Sh + il * Sl, y[ih][il]);
+        if (cx_writes(CTRL_HI, SCALED, ih, il)) st16<T, NT>(b + ih * Sh + il * Sl, y[ih][il]);
   }
 }
 
@@ -1296,10 +1323,14 @@ int launch_1q(dm_state *h, int q, const double *R) {
     const uint64_t S = pow4(p);
     const uint64_t units = h->N / 4 / W;
     int sh = 2 * p - (W == 2 ? 1 : 2);
-    if (use_i)
-      hipLaunchKernelGGL((k_1q_wide<T, true>), dim3(grid_for(h, units)), dim3(kBlock), 0, h->stream, a, units, sh, S, m);
-    else
-      hipLaunchKernelGGL((k_1q_wide<T, false>), dim3(grid_for(h, units)), dim3(kBlock), 0, h->stream, a, units, sh, S, m);
+    const dim3 grid(grid_for(h, units));
+    if (p >= 2) {  // runs of >= 16 coefficients: nontemporal
+      if (use_i) hipLaunchKernelGGL((k_1q_wide<T, true, true>), grid, dim3(kBlock), 0, h->stream, a, units, sh, S, m);
+      else hipLaunchKernelGGL((k_1q_wide<T, false, true>), grid, dim3(kBlock), 0, h->stream, a, units, sh, S, m);
+    } else {
+      if (use_i) hipLaunchKernelGGL((k_1q_wide<T, true, false>), grid, dim3(kBlock), 0, h->stream, a, units, sh, S, m);
+      else hipLaunchKernelGGL((k_1q_wide<T, false, false>), grid, dim3(kBlock), 0, h->stream, a, units, sh, S, m);
+    }
   }
   return last_launch("k_1q");
 }
@@ -1317,8 +1348,12 @@ int launch_cx_t(dm_state *h, int ph, int pl, const Scale16<T> &sc) {
   } else {
     const uint64_t units = h->N / 16 / W;
     const int sh_lo = 2 * pl - (W == 2 ? 1 : 2);
-    hipLaunchKernelGGL((k_cx_wide<T, CTRL_HI, SCALED>), dim3(grid_for(h, units)), dim3(kBlock), 0, h->stream, a,
-                       units, sh_lo, sh_mid, Sh, Sl, sc);
+    if (pl >= 2)  // runs of >= 16 coefficients: nontemporal
+      hipLaunchKernelGGL((k_cx_wide<T, CTRL_HI, SCALED, true>), dim3(grid_for(h, units)), dim3(kBlock), 0, h->stream,
+                         a, units, sh_lo, sh_mid, Sh, Sl, sc);
+    else
+      hipLaunchKernelGGL((k_cx_wide<T, CTRL_HI, SCALED, false>), dim3(grid_for(h, units)), dim3(kBlock), 0,
+                         h->stream, a, units, sh_lo, sh_mid, Sh, Sl, sc);
   }
   return last_launch("k_cx");
 }

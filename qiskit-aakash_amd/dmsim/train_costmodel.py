@@ -40,7 +40,7 @@ def time_program(prog, n, precision, reps=2):
     return dt
 
 
-def ab_compare(model, n=16, seeds=(5000, 5001, 5002), gates=400, precision='fp32'):
+def ab_compare(model, n=16, seeds=(5000, 5001, 5002, 5003, 5004), gates=400, precision='fp32'):
     """Measured seconds of learned vs greedy vs plain plans on held-out circuits."""
     out = []
     for s in seeds:
@@ -70,8 +70,20 @@ def main():
     ap.add_argument('--out', default=costmodel.PARAMS_FILE)
     ap.add_argument('--data', default=None, help='reuse a collected dataset (.npz)')
     ap.add_argument('--save-data', default=None)
+    ap.add_argument('--ab-only', action='store_true', help='re-run the A/B comparison with the saved model')
     args = ap.parse_args()
     n = args.qubits
+    if args.ab_only:
+        model = costmodel.CostModel.load(args.out)
+        meta = dict(model.meta)
+        meta['ab'] = ab_compare(model, n)
+        meta['time_weight'] = fusion.TIME_WEIGHT
+        model.meta = meta
+        model.save(args.out)
+        with open(os.path.splitext(args.out)[0] + '_report.json', 'w') as f:
+            json.dump(meta, f, indent=1)
+        print(json.dumps(meta['ab']), flush=True)
+        return
     t0 = time.perf_counter()
     if args.data:
         with np.load(args.data, allow_pickle=False) as z:
@@ -86,7 +98,7 @@ def main():
     params, report = costmodel.train(X, y, epochs=args.epochs)
     log('training: %s' % json.dumps(report))
     meta = dict(report, n=n, features=costmodel.N_FEATURES, hidden=costmodel.HIDDEN,
-                seeds='1000-2999', collect_seconds=t_collect)
+                seeds='1000-2999', collect_seconds=t_collect, time_weight=fusion.TIME_WEIGHT)
     model = costmodel.CostModel(params, 'fp32', meta)
     model.save(args.out)
     ab = ab_compare(model, n)
