@@ -687,14 +687,120 @@ __device__ __forceinline__ uint64_t tile_offset(int t, int w, int lomask, const 
   return off;
 }
 
+// The operations of a tile program on the LDS tile s (all threads of the
+// workgroup take part; a barrier follows each operation).
+template <typename T, int M, bool DENSE, bool GEN2Q, int NT>
+__device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, int nops, const T *__restrict__ prm) {
+  constexpr int E = 1 << (2 * M);
+  constexpr int G1 = (E / 4 + NT - 1) / NT;   // 1q groups per thread
+  constexpr int GCMAX = sizeof(T) == 8 ? 2 : 4;
+  constexpr int GC = G1 < GCMAX ? G1 : GCMAX;  // 1q groups per chunk
+  using Q = Pack<T, 4>;
+  const int tid = threadIdx.x;
+  for (int i = 0; i < nops; ++i) {
+    const TileOp op = ops[i];
+    const T *pv = prm + op.param;
+    if (op.kind == TOP_1Q) {
+      const int j = op.d[0];
+      T r[3][4];
+#pragma unroll
+      for (int x = 0; x < 3; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) r[x][y] = pv[4 * x + y];
+      // A thread's groups go in chunks of GC: all loads of a chunk are
+      // issued before its stores (LDS latency overlaps), and the chunk
+      // bounds the registers the op holds.
+      if (j == 0) {
+        // the four components of a group are adjacent: one vector LDS access
+#pragma unroll
+        for (int q0 = 0; q0 < G1; q0 += GC) {
+          Q x[GC];
+#pragma unroll
+          for (int q = 0; q < GC; ++q) {
+            const int g = tid + (q0 + q) * NT;
+            if (g < E / 4) x[q] = *reinterpret_cast<const Q *>(&s[4 * g]);
+          }
+#pragma unroll
+          for (int q = 0; q < GC; ++q) {
+            const int g = tid + (q0 + q) * NT;
+            if (g >= E / 4) continue;
+            Q y;
+            y.v[0] = x[q].v[0];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+              y.v[c + 1] = r[c][0] * x[q].v[0] + r[c][1] * x[q].v[1] + r[c][2] * x[q].v[2] + r[c][3] * x[q].v[3];
+            *reinterpret_cast<Q *>(&s[4 * g]) = y;
+          }
+        }
+      } else {
+        const int st = 1 << (2 * j);
+#pragma unroll
+        for (int q0 = 0; q0 < G1; q0 += GC) {
+          T x[GC][4];
+          int bb[GC];
+#pragma unroll
+          for (int q = 0; q < GC; ++q) {
+            const int g = tid + (q0 + q) * NT;
+            bb[q] = ins_zero(g, j);
+            if (g < E / 4) {
+#pragma unroll
+              for (int c = 0; c < 4; ++c) x[q][c] = s[bb[q] + c * st];
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < GC; ++q) {
+            const int g = tid + (q0 + q) * NT;
+            if (g >= E / 4) continue;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+              s[bb[q] + (c + 1) * st] =
+                  r[c][0] * x[q][0] + r[c][1] * x[q][1] + r[c][2] * x[q][2] + r[c][3] * x[q][3];
+          }
+        }
+      }
+    } else if (op.kind == TOP_CX || (GEN2Q && op.kind == TOP_2Q)) {
+      const int jc = op.d[0], jt = op.d[1];
+      const int jl = min(jc, jt), jh = max(jc, jt);
+      const int sc = 1 << (2 * jc), stt = 1 << (2 * jt);
+      for (int g = tid; g < E / 16; g += NT) {
+        const int b = ins_zero(ins_zero(g, jl), jh);
+        T v[4][4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+          for (int t = 0; t < 4; ++t) v[c][t] = s[b + c * sc + t * stt];
+        if (!GEN2Q || op.kind == TOP_CX) {
+          Scale16<T> sv;
+#pragma unroll
+          for (int q = 0; q < 16; ++q) sv.s[q] = op.k ? pv[q] : T(1);
+          cx_map<T, T, true, true>(v, sv);
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+              if (c | t) s[b + c * sc + t * stt] = v[c][t];
+        } else {
+#pragma unroll
+          for (int r = 1; r < 16; ++r) {
+            T acc = T(0);
+#pragma unroll
+            for (int c = 0; c < 16; ++c) acc += pv[16 * r + c] * v[c >> 2][c & 3];
+            s[b + (r >> 2) * sc + (r & 3) * stt] = acc;
+          }
+        }
+      }
+    } else if (DENSE && op.kind == TOP_DENSE) {
+      if (M == 6) tile_dense<T, M>(s, op, pv);
+    }
+    __syncthreads();
+  }
+}
+
 template <typename T, int M, bool DENSE, bool GEN2Q, int NT>
 __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
                                                       int nops, const T *__restrict__ prm, int remap) {
   constexpr int E = 1 << (2 * M);
   constexpr int QPT = (E / 4 + NT - 1) / NT;  // quads per thread (4 for M=6, 1 for M=5)
-  constexpr int G1 = (E / 4 + NT - 1) / NT;   // 1q groups per thread
-  constexpr int GCMAX = sizeof(T) == 8 ? 2 : 4;
-  constexpr int GC = G1 < GCMAX ? G1 : GCMAX;  // 1q groups per chunk
   using Q = Pack<T, 4>;
   __shared__ __align__(32) T s[E];
   const int tid = threadIdx.x;
@@ -743,109 +849,47 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
       }
     }
     __syncthreads();
-    for (int i = 0; i < nops; ++i) {
-      const TileOp op = ops[i];
-      const T *pv = prm + op.param;
-      if (op.kind == TOP_1Q) {
-        const int j = op.d[0];
-        T r[3][4];
-#pragma unroll
-        for (int x = 0; x < 3; ++x)
-#pragma unroll
-          for (int y = 0; y < 4; ++y) r[x][y] = pv[4 * x + y];
-        // A thread's groups go in chunks of GC: all loads of a chunk are
-        // issued before its stores (LDS latency overlaps), and the chunk
-        // bounds the registers the op holds.
-        if (j == 0) {
-          // the four components of a group are adjacent: one vector LDS access
-#pragma unroll
-          for (int q0 = 0; q0 < G1; q0 += GC) {
-            Q x[GC];
-#pragma unroll
-            for (int q = 0; q < GC; ++q) {
-              const int g = tid + (q0 + q) * NT;
-              if (g < E / 4) x[q] = *reinterpret_cast<const Q *>(&s[4 * g]);
-            }
-#pragma unroll
-            for (int q = 0; q < GC; ++q) {
-              const int g = tid + (q0 + q) * NT;
-              if (g >= E / 4) continue;
-              Q y;
-              y.v[0] = x[q].v[0];
-#pragma unroll
-              for (int c = 0; c < 3; ++c)
-                y.v[c + 1] = r[c][0] * x[q].v[0] + r[c][1] * x[q].v[1] + r[c][2] * x[q].v[2] + r[c][3] * x[q].v[3];
-              *reinterpret_cast<Q *>(&s[4 * g]) = y;
-            }
-          }
-        } else {
-          const int st = 1 << (2 * j);
-#pragma unroll
-          for (int q0 = 0; q0 < G1; q0 += GC) {
-            T x[GC][4];
-            int bb[GC];
-#pragma unroll
-            for (int q = 0; q < GC; ++q) {
-              const int g = tid + (q0 + q) * NT;
-              bb[q] = ins_zero(g, j);
-              if (g < E / 4) {
-#pragma unroll
-                for (int c = 0; c < 4; ++c) x[q][c] = s[bb[q] + c * st];
-              }
-            }
-#pragma unroll
-            for (int q = 0; q < GC; ++q) {
-              const int g = tid + (q0 + q) * NT;
-              if (g >= E / 4) continue;
-#pragma unroll
-              for (int c = 0; c < 3; ++c)
-                s[bb[q] + (c + 1) * st] =
-                    r[c][0] * x[q][0] + r[c][1] * x[q][1] + r[c][2] * x[q][2] + r[c][3] * x[q][3];
-            }
-          }
-        }
-      } else if (op.kind == TOP_CX || (GEN2Q && op.kind == TOP_2Q)) {
-        const int jc = op.d[0], jt = op.d[1];
-        const int jl = min(jc, jt), jh = max(jc, jt);
-        const int sc = 1 << (2 * jc), stt = 1 << (2 * jt);
-        for (int g = tid; g < E / 16; g += NT) {
-          const int b = ins_zero(ins_zero(g, jl), jh);
-          T v[4][4];
-#pragma unroll
-          for (int c = 0; c < 4; ++c)
-#pragma unroll
-            for (int t = 0; t < 4; ++t) v[c][t] = s[b + c * sc + t * stt];
-          if (!GEN2Q || op.kind == TOP_CX) {
-            Scale16<T> sv;
-#pragma unroll
-            for (int q = 0; q < 16; ++q) sv.s[q] = op.k ? pv[q] : T(1);
-            cx_map<T, T, true, true>(v, sv);
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-              for (int t = 0; t < 4; ++t)
-                if (c | t) s[b + c * sc + t * stt] = v[c][t];
-          } else {
-#pragma unroll
-            for (int r = 1; r < 16; ++r) {
-              T acc = T(0);
-#pragma unroll
-              for (int c = 0; c < 16; ++c) acc += pv[16 * r + c] * v[c >> 2][c & 3];
-              s[b + (r >> 2) * sc + (r & 3) * stt] = acc;
-            }
-          }
-        }
-      } else if (DENSE && op.kind == TOP_DENSE) {
-        if (M == 6) tile_dense<T, M>(s, op, pv);
-      }
-      __syncthreads();
-    }
+    tile_ops<T, M, DENSE, GEN2Q, NT>(s, ops, nops, prm);
 #pragma unroll
     for (int i = 0; i < QPT; ++i) {
       const int c = tid + i * NT;
       if (c < E / 4) stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), *reinterpret_cast<const Q *>(&s[4 * c]));
     }
     __syncthreads();
+  }
+}
+
+// One tile per workgroup (grid = tiles, tile = block index): no prefetch
+// buffer, so fewer registers and more resident workgroups per CU, which
+// overlap each other's HBM and LDS phases.
+template <typename T, int M, bool GEN2Q, int NT>
+__global__ __launch_bounds__(NT) void k_tile_stream(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
+                                                        int nops, const T *__restrict__ prm) {
+  constexpr int E = 1 << (2 * M);
+  constexpr int QPT = (E / 4 + NT - 1) / NT;
+  using Q = Pack<T, 4>;
+  __shared__ __align__(32) T s[E];
+  const uint64_t tile = blockIdx.x;
+  if (tile >= geo.tiles) return;  // uniform over the workgroup
+  const int tid = threadIdx.x;
+  const int w = geo.w;
+  const int lomask = (1 << (2 * w)) - 1;
+  uint64_t base = 0, r = tile;
+  for (int j = 0; j < geo.nr; ++j) {
+    base |= (r & 3ull) << (2 * geo.rpos[j]);
+    r >>= 2;
+  }
+#pragma unroll
+  for (int i = 0; i < QPT; ++i) {
+    const int c = tid + i * NT;
+    if (c < E / 4) *reinterpret_cast<Q *>(&s[4 * c]) = ldp<Q>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo));
+  }
+  __syncthreads();
+  tile_ops<T, M, false, GEN2Q, NT>(s, ops, nops, prm);
+#pragma unroll
+  for (int i = 0; i < QPT; ++i) {
+    const int c = tid + i * NT;
+    if (c < E / 4) stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), *reinterpret_cast<const Q *>(&s[4 * c]));
   }
 }
 
@@ -1836,6 +1880,19 @@ int tile_resident_per_cu() {
 }
 
 
+template <typename T, int MM, bool G, int NTH>
+int stream_resident_per_cu() {
+  static int cached = 0;
+  if (cached == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tile_stream<T, MM, G, NTH>, NTH, 0) != hipSuccess ||
+        nb < 1)
+      nb = 1;
+    cached = nb;
+  }
+  return cached;
+}
+
 template <typename T>
 int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T *d_vals) {
   ProfScope ps(h, 9);
@@ -1849,23 +1906,32 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
   // one variant per feature set keeps the register footprint of the common
   // 1q + CX passes small; 7-qubit tiles (64 / 128 KB of LDS) use 512 threads
   // Grid shape, from 16-qubit sweeps on MI355X:
-  //  - runs of >= 4^3 coefficients with several workgroups resident per CU:
-  //    one tile per workgroup in tile order (the hardware overlaps the
-  //    load, LDS and store phases of different workgroups);
-  //  - runs of 4^2 (the tiles of neighbouring indices share HBM lines) or a
-  //    single resident workgroup per CU (7-digit FP64 tiles): persistent
-  //    workgroups that prefetch their next tile, 8 (resp. 4) per resident
-  //    slot, with consecutive tiles kept on one XCD.
+  //  - FP32, runs of >= 4^3 coefficients, at least 4 workgroups of the
+  //    one-tile kernel resident per CU (6-digit tiles): k_tile_stream, one
+  //    tile per workgroup in tile order (the hardware overlaps the load, LDS
+  //    and store phases of different workgroups);
+  //  - other runs of >= 4^3 with 2 or more resident workgroups per CU:
+  //    k_tile_prog with one tile per workgroup, whose loads all issue before
+  //    the LDS writes;
+  //  - otherwise (runs of 4^2, whose neighbouring tiles share HBM lines, or
+  //    one resident workgroup per CU): persistent workgroups that prefetch
+  //    their next tile, 8 (resp. 4) per resident slot, with consecutive
+  //    tiles kept on one XCD.
 #define TILE_LAUNCH(MM, D, G, NTH)                                                                               \
   do {                                                                                                           \
     const int per_cu = tile_resident_per_cu<T, MM, D, G, NTH>();                                                \
-    const bool stream_tiles = fp.geo.w >= 3 && per_cu >= 2;                                                      \
+    const bool long_runs = fp.geo.w >= 3 && fp.geo.tiles <= (1ull << 30);                                     \
+    if (long_runs && !D && sizeof(T) == 4 && stream_resident_per_cu<T, MM, G, NTH>() >= 4) {                   \
+      hipLaunchKernelGGL((k_tile_stream<T, MM, G, NTH>), dim3((unsigned)fp.geo.tiles), dim3(NTH), 0, h->stream,  \
+                         a, fp.geo, d_ops, nops, d_vals);                                                        \
+      break;                                                                                                     \
+    }                                                                                                            \
+    const bool full = long_runs && per_cu >= 2;                                                                  \
     const uint64_t slots = (uint64_t)h->cu_count * per_cu * (per_cu >= 2 ? 8 : 4);                              \
-    uint64_t g = stream_tiles ? fp.geo.tiles : std::min<uint64_t>(fp.geo.tiles, slots);                          \
-    if (g >= 8) g -= g % 8;                                                                                      \
-    g = std::min<uint64_t>(g, 1ull << 30);                                                                       \
+    uint64_t g = full ? fp.geo.tiles : std::min<uint64_t>(fp.geo.tiles, slots);                                 \
+    if (!full && g >= 8) g -= g % 8;                                                                             \
     hipLaunchKernelGGL((k_tile_prog<T, MM, D, G, NTH>), dim3((unsigned)g), dim3(NTH), 0, h->stream, a, fp.geo,   \
-                       d_ops, nops, d_vals, stream_tiles ? 0 : 1);                                               \
+                       d_ops, nops, d_vals, full ? 0 : 1);                                                       \
   } while (0)
   if (fp.m == 7 && gen) TILE_LAUNCH(7, false, true, 512);
   else if (fp.m == 7) TILE_LAUNCH(7, false, false, 512);
