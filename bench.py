@@ -247,15 +247,20 @@ def kernels_mode(args):
 
 
 def batch_mode(args):
+    """Batched circuits (U3/CX 70/30, memory noise after every gate, Z
+    ensemble at the end), FP64 and FP32, autotuned, hipGraph replay."""
     from dmsim import batch
-    if args.batch_circuits > 64:
-        # 512 circuits of 6-12 qubits enqueue ~4e5 kernel launches per run; a
-        # run of that batch ended the process on MI355X without an error
-        # message, and it stays off until the cause is known
-        print('batch config: %d circuits refused (at most 64 run on this build)' % args.batch_circuits,
-              file=sys.stderr)
+    lo, hi = (int(x) for x in args.batch_widths.split('-'))
+    count = args.batch_circuits
+    if hi > 7 and count > 12:
+        # Circuits above 7 qubits keep their state in HBM and run one kernel
+        # per operation: 512 circuits of 6-12 qubits enqueue ~4.8e5 launches
+        # per run.  A run of that batch ended the process on MI355X without
+        # an error message, and it stays off until the cause is known.
+        print('batch config: %d circuits with widths up to %d refused (at most 12 when the batch holds circuits of '
+              'more than 7 qubits)' % (count, hi), file=sys.stderr)
         sys.exit(3)
-    specs = workloads.batch(args.batch_circuits, args.seed)
+    specs = workloads.batch(count, args.seed, n_range=(lo, hi))
     lines = {}
     for prec in ('fp64', 'fp32'):
         (lmax, thr), tune = batch.autotune(specs, prec)
@@ -270,16 +275,16 @@ def batch_mode(args):
         wall = time.perf_counter() - t0
         b.close()
         us = np.concatenate(us_all)
-        lines[prec] = {'circuits_per_s': 512 * args.steps / wall, 'p50_us': float(np.percentile(us, 50)),
+        lines[prec] = {'circuits_per_s': count * args.steps / wall, 'p50_us': float(np.percentile(us, 50)),
                        'p99_us': float(np.percentile(us, 99)), 'lds_max_qubits': lmax, 'lds_threads': thr,
                        'autotune_ms': tune.get('trials_ms')}
         log('%s: %s' % (prec, json.dumps(lines[prec])))
-    return {'metric': 'circuits/sec, 512 noisy circuits of 6-12 qubits (one hipGraph launch)',
+    return {'metric': 'circuits/sec, %d noisy circuits of %d-%d qubits (one hipGraph launch)' % (count, lo, hi),
             'value': lines['fp64']['circuits_per_s'], 'unit': 'circuits/s', 'n_gpus': 1, 'steps': args.steps,
             'warmup': args.warmup, 'higher_is_better': True, 'dtype': 'f64', 'data': 'synthetic',
-            'config': {'workload': '512 circuits, widths U{6..12}, gates U{50..300} (U3 0.7 / CX 0.3), decay and '
+            'config': {'workload': '%d circuits, widths U{%d..%d}, gates U{50..300} (U3 0.7 / CX 0.3), decay and '
                                    'depolarisation rates logU[1e-4,1e-2], barrier after every gate, seed %d'
-                                   % args.seed, 'fp64': lines['fp64'], 'fp32': lines['fp32']}}
+                                   % (count, lo, hi, args.seed), 'fp64': lines['fp64'], 'fp32': lines['fp32']}}
 
 
 def precision_mode(args):
@@ -370,6 +375,7 @@ def main():
     ap.add_argument('--layers', type=int, default=400, help='precision config: layers')
     ap.add_argument('--kernels', action='store_true', help='short per-kernel run (for rocprofv3 counters)')
     ap.add_argument('--batch-circuits', type=int, default=512, help='batch config: circuits')
+    ap.add_argument('--batch-widths', default='6-12', help='batch config: qubit range lo-hi')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU: the path has no multi-GPU form in this build (--gpus %d refused)'

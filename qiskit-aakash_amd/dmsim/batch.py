@@ -23,6 +23,10 @@ from .program import Program
 
 _TUNE_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'batch_tuning.json')
 
+# circuits above the LDS cutoff run one kernel per operation; a graph-captured
+# run holds at most 16384 launches (dm_batch_run), about a dozen such circuits
+MAX_HBM_CIRCUITS = 12
+
 
 def circuit_program(n, gates, decay_factor=1.0, decoherence_factor=1.0, thermal_factor=1.0,
                     gate_depolarization=0.0):
@@ -39,6 +43,24 @@ def circuit_program(n, gates, decay_factor=1.0, decoherence_factor=1.0, thermal_
         else:
             raise ValueError('batch circuits hold u1/u3/cx gates, got %s' % name)
         prog.memory_noise(decoherence_factor, thermal_factor, decay_factor)
+    return prog
+
+
+def fused_circuit_program(n, gates, decay_factor=1.0, decoherence_factor=1.0, thermal_factor=1.0,
+                          gate_depolarization=0.0):
+    """The same circuit as ``circuit_program`` as fused tile passes: the
+    memory noise of every level enters the gate stream as one 1-qubit
+    channel per qubit, and the greedy planner groups everything into blocks
+    of at most 4 qubits (registers of at least 6 qubits)."""
+    from . import fusion
+    noisy = []
+    mem = ptm.memory_noise(decoherence_factor, thermal_factor, decay_factor)
+    identity = decay_factor == 1.0 and decoherence_factor == 1.0
+    for gate in gates:
+        noisy += fusion.noisy_gates([gate], gate_depolarization)
+        if not identity:
+            noisy += [fusion.NoisyGate((q,), mem, 'u3') for q in range(n)]
+    prog, _ = fusion.fused_program(noisy, n, fusion.KMAX['greedy'], 'greedy', None, 'seq')
     return prog
 
 
@@ -102,16 +124,22 @@ class Batch:
             pass
 
 
-def from_workload(specs, precision='fp64', **kw):
+def from_workload(specs, precision='fp64', lds_max_qubits=7, **kw):
     """Batch from dmsim.workloads.batch specs (n, gates, decay_rate, depol_rate):
-    decay_factor = 1 - decay_rate, measurement retention = 1 - depol_rate."""
-    circuits = [(n, circuit_program(n, gates, decay_factor=1.0 - dr), 1.0 - pr) for n, gates, dr, pr in specs]
-    return Batch(circuits, precision, **kw)
+    decay_factor = 1 - decay_rate, measurement retention = 1 - depol_rate.
+    Circuits above the LDS cutoff (and of at least 6 qubits) run as fused
+    tile passes."""
+    circuits = []
+    for n, gates, dr, pr in specs:
+        build = fused_circuit_program if (n > lds_max_qubits and n >= 6) else circuit_program
+        circuits.append((n, build(n, gates, decay_factor=1.0 - dr), 1.0 - pr))
+    return Batch(circuits, precision, lds_max_qubits=lds_max_qubits, **kw)
 
 
 def autotune(specs, precision='fp64', reps=2, use_cache=True):
     """Best (lds_max_qubits, lds_threads) for this batch by measured time."""
-    key = '%s:%s' % (precision, native.device_info(0)['cu_count'])
+    ns = [sp[0] for sp in specs]
+    key = '%s:%s:%d:%d-%d' % (precision, native.device_info(0)['cu_count'], len(specs), min(ns), max(ns))
     cache = {}
     if use_cache and os.path.exists(_TUNE_CACHE):
         try:
@@ -121,7 +149,11 @@ def autotune(specs, precision='fp64', reps=2, use_cache=True):
         if key in cache:
             return tuple(cache[key]['choice']), cache[key]
     trials = {}
-    for lmax, thr in itertools.product((5, 6, 7), (256, 512, 1024)):
+    ns = [sp[0] for sp in specs]
+    # a cutoff that leaves more than MAX_HBM_CIRCUITS circuits in HBM is not
+    # a candidate (their per-operation launches exceed the graph limit)
+    lmaxes = [l for l in (5, 6, 7) if sum(1 for n in ns if n > l) <= MAX_HBM_CIRCUITS] or [7]
+    for lmax, thr in itertools.product(lmaxes, (256, 512, 1024)):
         b = from_workload(specs, precision, lds_max_qubits=lmax, lds_threads=thr)
         b.run(True)
         t = min(b.run(True)[2] for _ in range(reps))
