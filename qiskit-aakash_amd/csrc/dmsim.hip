@@ -863,7 +863,7 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
 // buffer, so fewer registers and more resident workgroups per CU, which
 // overlap each other's HBM and LDS phases.
 template <typename T, int M, bool GEN2Q, int NT>
-__global__ __launch_bounds__(NT) void k_tile_stream(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 1))) void k_tile_stream(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
                                                         int nops, const T *__restrict__ prm) {
   constexpr int E = 1 << (2 * M);
   constexpr int QPT = (E / 4 + NT - 1) / NT;
@@ -1906,10 +1906,12 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
   // one variant per feature set keeps the register footprint of the common
   // 1q + CX passes small; 7-qubit tiles (64 / 128 KB of LDS) use 512 threads
   // Grid shape, from 16-qubit sweeps on MI355X:
-  //  - FP32, runs of >= 4^3 coefficients, at least 4 workgroups of the
-  //    one-tile kernel resident per CU (6-digit tiles): k_tile_stream, one
-  //    tile per workgroup in tile order (the hardware overlaps the load, LDS
-  //    and store phases of different workgroups);
+  //  - FP32, runs of >= 4^3 coefficients, at least 28 waves of the
+  //    one-tile kernel resident per CU (6-digit tiles: 64 VGPRs, 8
+  //    workgroups of 256 threads; 7-digit tiles stay below, their 64 KB of
+  //    LDS allows 2): k_tile_stream, one tile per workgroup in tile order
+  //    (the hardware overlaps the load, LDS and store phases of different
+  //    workgroups);
   //  - other runs of >= 4^3 with 2 or more resident workgroups per CU:
   //    k_tile_prog with one tile per workgroup, whose loads all issue before
   //    the LDS writes;
@@ -1921,7 +1923,7 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
   do {                                                                                                           \
     const int per_cu = tile_resident_per_cu<T, MM, D, G, NTH>();                                                \
     const bool long_runs = fp.geo.w >= 3 && fp.geo.tiles <= (1ull << 30);                                     \
-    if (long_runs && !D && sizeof(T) == 4 && stream_resident_per_cu<T, MM, G, NTH>() >= 4) {                   \
+    if (long_runs && !D && sizeof(T) == 4 && stream_resident_per_cu<T, MM, G, NTH>() * NTH / 64 >= 28) {       \
       hipLaunchKernelGGL((k_tile_stream<T, MM, G, NTH>), dim3((unsigned)fp.geo.tiles), dim3(NTH), 0, h->stream,  \
                          a, fp.geo, d_ops, nops, d_vals);                                                        \
       break;                                                                                                     \
