@@ -613,11 +613,45 @@ struct TileOp {
 struct TileGeom {
   int m;                  // tile digits (== template M)
   int w;                  // leading digits with tpos[j] == j
-  int tpos[8];  // up to 7 tile digits
+  int tpos[8];            // HBM position of each tile digit (ascending)
   int nr;
   int rpos[kMaxQubits];
   uint64_t tiles;
+  // LDS layout: tile digit j sits at LDS digit lsh[j] / 2.  The digits the
+  // operations act on are placed above the others, so that the strided LDS
+  // accesses of an operation fall on distinct banks; ident marks the
+  // identity layout.
+  int lsh[8];
+  int ident;
 };
+
+// LDS index of tile element 4c (the quad of tile digit 0 starting there)
+// under the layout of geo; the quad's elements are 1 << geo.lsh[0] apart.
+template <int M>
+__device__ __forceinline__ int lds_quad_base(int c, const TileGeom &geo) {
+  if (geo.ident) return 4 * c;
+  int base = 0;
+#pragma unroll
+  for (int j = 1; j < M; ++j) base |= ((c >> (2 * (j - 1))) & 3) << geo.lsh[j];
+  return base;
+}
+template <typename T>
+__device__ __forceinline__ void lds_put_quad(T *s, int lb, int le, const Pack<T, 4> &q) {
+  if (le == 0) {
+    *reinterpret_cast<Pack<T, 4> *>(&s[lb]) = q;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) s[lb + (e << le)] = q.v[e];
+  }
+}
+template <typename T>
+__device__ __forceinline__ Pack<T, 4> lds_get_quad(const T *s, int lb, int le) {
+  if (le == 0) return *reinterpret_cast<const Pack<T, 4> *>(&s[lb]);
+  Pack<T, 4> q;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) q.v[e] = s[lb + (e << le)];
+  return q;
+}
 
 __device__ __forceinline__ int ins_zero(int g, int j) {
   const int lo = g & ((1 << (2 * j)) - 1);
@@ -796,7 +830,7 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
   }
 }
 
-template <typename T, int M, bool DENSE, bool GEN2Q, int NT>
+template <typename T, int M, bool DENSE, bool GEN2Q, int NT, bool LAYOUT>
 __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
                                                       int nops, const T *__restrict__ prm, int remap) {
   constexpr int E = 1 << (2 * M);
@@ -820,7 +854,12 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
     }
     return base;
   };
-  // quads of 4 consecutive tile indices are 4 consecutive coefficients
+  // quads of 4 consecutive tile indices are 4 consecutive coefficients; their
+  // LDS places are the same for every tile
+  int lb[QPT];
+#pragma unroll
+  for (int i = 0; i < QPT; ++i) lb[i] = LAYOUT ? lds_quad_base<M>(tid + i * NT, geo) : 4 * (tid + i * NT);
+  const int le = LAYOUT ? geo.lsh[0] : 0;
   Q buf[QPT];
   uint64_t tile = lid;
   if (tile < geo.tiles) {
@@ -836,7 +875,7 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
 #pragma unroll
     for (int i = 0; i < QPT; ++i) {
       const int c = tid + i * NT;
-      if (c < E / 4) *reinterpret_cast<Q *>(&s[4 * c]) = buf[i];
+      if (c < E / 4) lds_put_quad(s, lb[i], le, buf[i]);
     }
     // prefetch the next tile while this one is computed
     const uint64_t next = tile + gridDim.x;
@@ -853,7 +892,7 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
 #pragma unroll
     for (int i = 0; i < QPT; ++i) {
       const int c = tid + i * NT;
-      if (c < E / 4) stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), *reinterpret_cast<const Q *>(&s[4 * c]));
+      if (c < E / 4) stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), lds_get_quad(s, lb[i], le));
     }
     __syncthreads();
   }
@@ -862,7 +901,7 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
 // One tile per workgroup (grid = tiles, tile = block index): no prefetch
 // buffer, so fewer registers and more resident workgroups per CU, which
 // overlap each other's HBM and LDS phases.
-template <typename T, int M, bool GEN2Q, int NT>
+template <typename T, int M, bool GEN2Q, int NT, bool LAYOUT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 1))) void k_tile_stream(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
                                                         int nops, const T *__restrict__ prm) {
   constexpr int E = 1 << (2 * M);
@@ -882,14 +921,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
 #pragma unroll
   for (int i = 0; i < QPT; ++i) {
     const int c = tid + i * NT;
-    if (c < E / 4) *reinterpret_cast<Q *>(&s[4 * c]) = ldp<Q>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo));
+    if (c < E / 4)
+      lds_put_quad(s, LAYOUT ? lds_quad_base<M>(c, geo) : 4 * c, LAYOUT ? geo.lsh[0] : 0,
+                   ldp<Q>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo)));
   }
   __syncthreads();
   tile_ops<T, M, false, GEN2Q, NT>(s, ops, nops, prm);
 #pragma unroll
   for (int i = 0; i < QPT; ++i) {
     const int c = tid + i * NT;
-    if (c < E / 4) stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), *reinterpret_cast<const Q *>(&s[4 * c]));
+    if (c < E / 4)
+      stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo),
+          lds_get_quad(s, LAYOUT ? lds_quad_base<M>(c, geo) : 4 * c, LAYOUT ? geo.lsh[0] : 0));
   }
 }
 
@@ -1861,17 +1904,48 @@ int parse_fused(const dm_state *h, const double *p, int64_t avail, int opi, Fuse
     at += nv;
     fp->ops.push_back(op);
   }
+  // LDS layout.  A two-qubit operation on tile digits 0-2 makes the lanes of
+  // a wave hit 4-16 times the same LDS bank (measured: a CX on digits 0 and 1
+  // costs 3.5x a CX on digits 3 and 4).  Then the digits of two-qubit
+  // operations go to the top LDS digits and the others below, in ascending
+  // order; single-qubit operations on low digits are cheap in place (the
+  // lowest digit has a vector path) and do not move digits, since a moved
+  // digit makes the HBM <-> LDS copy itself conflict.  Operation digits are
+  // rewritten to LDS digits.
+  bool used[8] = {false}, low2q = false, dense = false;
+  for (const TileOp &op : fp->ops) {
+    dense = dense || op.kind == TOP_DENSE;
+    if (op.kind != TOP_CX && op.kind != TOP_2Q) continue;
+    used[op.d[0]] = used[op.d[1]] = true;
+    low2q = low2q || op.d[0] < 3 || op.d[1] < 3;
+  }
+  if (dense || !low2q)  // matrix-core passes and high operations keep the identity layout
+    for (int d = 0; d < 8; ++d) used[d] = false;
+  int lds_digit[8], nl = 0;
+  for (int d = 0; d < m; ++d)
+    if (!used[d]) lds_digit[d] = nl++;
+  for (int d = 0; d < m; ++d)
+    if (used[d]) lds_digit[d] = nl++;
+  fp->geo.ident = 1;
+  for (int d = 0; d < m; ++d) {
+    fp->geo.lsh[d] = 2 * lds_digit[d];
+    fp->geo.ident = fp->geo.ident && lds_digit[d] == d;
+  }
+  for (TileOp &op : fp->ops) {
+    const int nd = op.kind == TOP_1Q ? 1 : (op.kind == TOP_DENSE ? op.k : 2);
+    for (int d = 0; d < nd; ++d) op.d[d] = lds_digit[op.d[d]];
+  }
   return DM_OK;
 }
 
 // Workgroups of one tile-program variant that fit on a CU at once (registers,
 // LDS, waves), from the runtime's occupancy calculator; cached per variant.
-template <typename T, int MM, bool D, bool G, int NTH>
+template <typename T, int MM, bool D, bool G, int NTH, bool L>
 int tile_resident_per_cu() {
   static int cached = 0;
   if (cached == 0) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tile_prog<T, MM, D, G, NTH>, NTH, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tile_prog<T, MM, D, G, NTH, L>, NTH, 0) != hipSuccess ||
         nb < 1)
       nb = 1;
     cached = nb;
@@ -1880,12 +1954,12 @@ int tile_resident_per_cu() {
 }
 
 
-template <typename T, int MM, bool G, int NTH>
+template <typename T, int MM, bool G, int NTH, bool L>
 int stream_resident_per_cu() {
   static int cached = 0;
   if (cached == 0) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tile_stream<T, MM, G, NTH>, NTH, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tile_stream<T, MM, G, NTH, L>, NTH, 0) != hipSuccess ||
         nb < 1)
       nb = 1;
     cached = nb;
@@ -1919,29 +1993,35 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
   //    one resident workgroup per CU): persistent workgroups that prefetch
   //    their next tile, 8 (resp. 4) per resident slot, with consecutive
   //    tiles kept on one XCD.
-#define TILE_LAUNCH(MM, D, G, NTH)                                                                               \
+#define TILE_LAUNCH_L(MM, D, G, NTH, L)                                                                          \
   do {                                                                                                           \
-    const int per_cu = tile_resident_per_cu<T, MM, D, G, NTH>();                                                \
+    const int per_cu = tile_resident_per_cu<T, MM, D, G, NTH, L>();                                             \
     const bool long_runs = fp.geo.w >= 3 && fp.geo.tiles <= (1ull << 30);                                     \
-    if (long_runs && !D && sizeof(T) == 4 && stream_resident_per_cu<T, MM, G, NTH>() * NTH / 64 >= 28) {       \
-      hipLaunchKernelGGL((k_tile_stream<T, MM, G, NTH>), dim3((unsigned)fp.geo.tiles), dim3(NTH), 0, h->stream,  \
-                         a, fp.geo, d_ops, nops, d_vals);                                                        \
+    if (long_runs && !D && sizeof(T) == 4 && stream_resident_per_cu<T, MM, G, NTH, L>() * NTH / 64 >= 28) {    \
+      hipLaunchKernelGGL((k_tile_stream<T, MM, G, NTH, L>), dim3((unsigned)fp.geo.tiles), dim3(NTH), 0,          \
+                         h->stream, a, fp.geo, d_ops, nops, d_vals);                                             \
       break;                                                                                                     \
     }                                                                                                            \
     const bool full = long_runs && per_cu >= 2;                                                                  \
     const uint64_t slots = (uint64_t)h->cu_count * per_cu * (per_cu >= 2 ? 8 : 4);                              \
     uint64_t g = full ? fp.geo.tiles : std::min<uint64_t>(fp.geo.tiles, slots);                                 \
     if (!full && g >= 8) g -= g % 8;                                                                             \
-    hipLaunchKernelGGL((k_tile_prog<T, MM, D, G, NTH>), dim3((unsigned)g), dim3(NTH), 0, h->stream, a, fp.geo,   \
-                       d_ops, nops, d_vals, full ? 0 : 1);                                                       \
+    hipLaunchKernelGGL((k_tile_prog<T, MM, D, G, NTH, L>), dim3((unsigned)g), dim3(NTH), 0, h->stream, a,       \
+                       fp.geo, d_ops, nops, d_vals, full ? 0 : 1);                                               \
+  } while (0)
+#define TILE_LAUNCH(MM, D, G, NTH)               \
+  do {                                           \
+    if (fp.geo.ident) TILE_LAUNCH_L(MM, D, G, NTH, false); \
+    else TILE_LAUNCH_L(MM, D, G, NTH, true);     \
   } while (0)
   if (fp.m == 7 && gen) TILE_LAUNCH(7, false, true, 512);
   else if (fp.m == 7) TILE_LAUNCH(7, false, false, 512);
-  else if (fp.m == 6 && dense) TILE_LAUNCH(6, true, true, 256);
+  else if (fp.m == 6 && dense) TILE_LAUNCH_L(6, true, true, 256, false);
   else if (fp.m == 6 && gen) TILE_LAUNCH(6, false, true, 256);
   else if (fp.m == 6) TILE_LAUNCH(6, false, false, 256);
   else if (gen) TILE_LAUNCH(5, false, true, 256);
   else TILE_LAUNCH(5, false, false, 256);
+#undef TILE_LAUNCH_L
 #undef TILE_LAUNCH
   return last_launch("k_tile_prog");
 }

@@ -197,9 +197,9 @@ def candidate_blocks(gates, done, start, kmax, n):
     return list(cands.values())
 
 
-def min_block_qubits(n):
-    """The MFMA block kernel needs n >= k + column qubits (3+3 or 4+2)."""
-    return 6
+# blocks of 3-4 qubits run as tile passes of at least 6 digits (the MFMA form
+# needs k + column digits = 3+3 or 4+2), so smaller registers use k <= 2
+MIN_TILE_QUBITS = 6
 
 
 def plan(gates, n, kmax=4, method='greedy', cost=None):
@@ -207,7 +207,7 @@ def plan(gates, n, kmax=4, method='greedy', cost=None):
     done = [False] * len(gates)
     blocks = []
     first = 0
-    allow_mfma = n >= min_block_qubits(n)
+    allow_mfma = n >= MIN_TILE_QUBITS
     spent = [0.0, 0]  # predicted seconds and gates of the blocks chosen so far
     while True:
         while first < len(gates) and done[first]:

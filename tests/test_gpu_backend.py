@@ -207,6 +207,22 @@ def test_fp32_storage_against_fp64_oracle(n):
     assert pe <= 1e-5
 
 
+@pytest.mark.parametrize('n', [4, 9])
+def test_fused_fp32_execution_against_fp64_oracle(n):
+    """Learned fusion with FP32 storage through execute(): small registers
+    (n < 6: blocks of at most 2 qubits) and tile passes (n = 9)."""
+    gates = workloads.local_u3_cx(n, 150, 21)
+    qc = workloads.to_circuit(n, gates, barrier_after_each=True, measure=('ensemble', 'Z'))
+    opts = {'decay_factor': 0.995, 'decoherence_factor': 0.99, 'thermal_factor': 0.8,
+            'depolarization_factor': 0.999, 'gate_depolarization': 0.001, 'compute_densitymatrix': False}
+    ref, got, _ = run_both(qc, opts, precision='fp32', fusion='learned')
+    err = np.max(np.abs(ref['coeffmatrix'] - got['coeffmatrix']))
+    assert err * 2 ** n <= 2e-4
+    assert got['coeffmatrix'][0] == 2.0 ** -n
+    pe = max(abs(ref['ensemble_probability'][k] - got['ensemble_probability'][k]) for k in ref['ensemble_probability'])
+    assert pe <= 1e-5
+
+
 def test_sticky_options_mirror_reference():
     """compute_densitymatrix and custom_densitymatrix persist on the backend
     between runs, as on the reference's cached backend instance."""

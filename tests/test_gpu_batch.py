@@ -44,3 +44,20 @@ def test_batch_split_choices_agree(lmax, threads):
     b.close()
     for g, w in zip(got, want):
         assert np.max(np.abs(g - w)) <= 1e-13
+
+
+@pytest.mark.parametrize('precision,tol', [('fp64', 1e-12), ('fp32', 2e-5)])
+def test_fused_programs_in_lds_and_hbm_tiers(precision, tol):
+    """Fused tile programs (memory noise as per-qubit channels) as batch
+    circuits: inlined into the LDS kernel for 6-7 qubits, tile passes in HBM
+    for 8 qubits."""
+    specs = workloads.batch(6, 300, n_range=(6, 8), depth_range=(20, 60))
+    circuits = [(n, batch.fused_circuit_program(n, gates, decay_factor=1.0 - dr), 1.0 - pr)
+                for n, gates, dr, pr in specs]
+    b = batch.Batch(circuits, precision, lds_max_qubits=7, lds_threads=256)
+    probs, us, ms = b.run(True)
+    b.close()
+    assert any(n <= 7 for n, _, _, _ in specs) and any(n == 8 for n, _, _, _ in specs)
+    for (n, gates, dr, pr), got in zip(specs, probs):
+        want = oracle_probs(n, gates, dr, pr)
+        assert np.max(np.abs(got - want)) <= tol, n
