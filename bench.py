@@ -246,6 +246,36 @@ def kernels_mode(args):
     return out
 
 
+def dense_kernels_mode(args):
+    """One short run of the matrix-core kernels at n qubits (for rocprofv3
+    MFMA counter passes): per precision, the standalone block kernel
+    (dm_apply_block) and the dense tile pass, each for a 3- and a 4-qubit
+    block PTM, 2 launches each, in this order."""
+    from dmsim.program import Program
+    n = args.qubits
+    rng = np.random.default_rng(args.seed)
+    out = {'n': n, 'order': []}
+    for prec in ('fp32', 'fp64'):
+        st = native.DeviceState(n, prec)
+        zero_state(st, n)
+        for k in (3, 4):
+            qs = list(range(k))
+            S = np.eye(4 ** k)
+            S[1:, 1:] = rng.normal(size=(4 ** k - 1, 4 ** k - 1)) / 4 ** k
+            for form in ('block', 'tile'):
+                prog = Program(n)
+                for _ in range(2):
+                    if form == 'block':
+                        prog.block(qs, S)
+                    else:
+                        prog.fused(6, qs, [(native.SUB_DENSE, qs, S)])
+                prog.run(st)
+                st.synchronize()
+                out['order'].append('%s_%s_k%d' % (prec, form, k))
+        st.close()
+    return out
+
+
 def batch_mode(args):
     """Batched circuits (U3/CX 70/30, memory noise after every gate, Z
     ensemble at the end), FP64 and FP32, autotuned, hipGraph replay."""
@@ -374,6 +404,7 @@ def main():
     ap.add_argument('--config', default='headline', choices=['headline', 'batch', 'precision', 'thermal'])
     ap.add_argument('--layers', type=int, default=400, help='precision config: layers')
     ap.add_argument('--kernels', action='store_true', help='short per-kernel run (for rocprofv3 counters)')
+    ap.add_argument('--kernels-dense', action='store_true', help='short matrix-core kernel run (MFMA counters)')
     ap.add_argument('--batch-circuits', type=int, default=512, help='batch config: circuits')
     ap.add_argument('--batch-widths', default='6-12', help='batch config: qubit range lo-hi')
     args = ap.parse_args()
@@ -384,6 +415,9 @@ def main():
 
     if args.kernels:
         print(json.dumps(kernels_mode(args)), flush=True)
+        return
+    if args.kernels_dense:
+        print(json.dumps(dense_kernels_mode(args)), flush=True)
         return
     if args.config != 'headline':
         fn = {'batch': batch_mode, 'precision': precision_mode, 'thermal': thermal_mode}[args.config]
