@@ -331,6 +331,40 @@ def test_fused_passes_with_dense_two_qubit_ops(prec):
     assert np.max(np.abs(got - want)) <= t
 
 
+@pytest.mark.parametrize('prec', PRECS)
+@pytest.mark.parametrize('m,qs', [(6, [8, 7, 0]), (6, [8, 6, 2, 1]), (7, [8, 7, 3, 0]), (5, [8, 7])])
+def test_fused_pass_two_qubit_ops_on_low_digits(prec, m, qs):
+    """CX and dense two-qubit operations on the lowest tile digits move those
+    digits to the top of the LDS tile (bank-conflict-free layout); results
+    must equal one kernel per gate."""
+    from dmsim.program import Program
+    n = 9
+    rng = np.random.default_rng(len(qs) + m)
+    a = rand_state(n, rng)
+    R = ptm.rows3(ptm.gate_ptm('u3', [0.4, 1.1, -0.3]))
+    scale = ptm.depolarizing_pair_scale(2e-3)
+    tsp = ptm.cx_ptm((0.97, 0.05))
+    sub, plain = [], Program(n)
+    for i in range(len(qs)):
+        c, t = qs[i], qs[(i + 1) % len(qs)]
+        sub += [(native.SUB_1Q, [c], R), (native.SUB_CX, [c, t], scale), (native.SUB_2Q, [t, c], tsp)]
+        plain.one_qubit(c, ptm.gate_ptm('u3', [0.4, 1.1, -0.3]))
+        plain.cx(c, t, scale)
+        plain.two_qubit(t, c, tsp)
+    fused = Program(n)
+    fused.fused(m, qs, sub)
+    s1 = make(n, 'fp64', a)
+    plain.run(s1)
+    want = s1.download()
+    s1.close()
+    s2 = make(n, prec, a)
+    fused.run(s2)
+    got = s2.download()
+    s2.close()
+    t = 1e-14 * 2.0 ** -n * 50 if prec == 'fp64' else 2e-6 * 2.0 ** -n * 50
+    assert np.max(np.abs(got - want)) <= t
+
+
 def test_fused_pass_rejects_bad_records():
     from dmsim.program import Program
     st = native.DeviceState(6, 'fp64')
