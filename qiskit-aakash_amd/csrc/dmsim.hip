@@ -623,6 +623,11 @@ struct TileGeom {
   // identity layout.
   int lsh[8];
   int ident;
+  // Under a moved layout, quad c of the HBM <-> LDS copy is the tile index
+  // with digit qord[k] = base-4 digit k of c: consecutive lanes then vary the
+  // tile digits of the lowest LDS digits (distinct banks), at the price of
+  // shorter HBM segments per lane.
+  int qord[8];
 };
 
 // LDS index of tile element 4c (the quad of tile digit 0 starting there)
@@ -632,8 +637,17 @@ __device__ __forceinline__ int lds_quad_base(int c, const TileGeom &geo) {
   if (geo.ident) return 4 * c;
   int base = 0;
 #pragma unroll
-  for (int j = 1; j < M; ++j) base |= ((c >> (2 * (j - 1))) & 3) << geo.lsh[j];
+  for (int k = 0; k < M - 1; ++k) base |= ((c >> (2 * k)) & 3) << geo.lsh[geo.qord[k]];
   return base;
+}
+// Tile index (digit 0 zero) of quad c of the copy.
+template <int M>
+__device__ __forceinline__ int quad_tile_index(int c, const TileGeom &geo) {
+  if (geo.ident) return 4 * c;
+  int t = 0;
+#pragma unroll
+  for (int k = 0; k < M - 1; ++k) t |= ((c >> (2 * k)) & 3) << (2 * geo.qord[k]);
+  return t;
 }
 template <typename T>
 __device__ __forceinline__ void lds_put_quad(T *s, int lb, int le, const Pack<T, 4> &q) {
@@ -859,6 +873,9 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
   int lb[QPT];
 #pragma unroll
   for (int i = 0; i < QPT; ++i) lb[i] = LAYOUT ? lds_quad_base<M>(tid + i * NT, geo) : 4 * (tid + i * NT);
+  int tq[QPT];
+#pragma unroll
+  for (int i = 0; i < QPT; ++i) tq[i] = LAYOUT ? quad_tile_index<M>(tid + i * NT, geo) : 4 * (tid + i * NT);
   const int le = LAYOUT ? geo.lsh[0] : 0;
   Q buf[QPT];
   uint64_t tile = lid;
@@ -867,7 +884,7 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
 #pragma unroll
     for (int i = 0; i < QPT; ++i) {
       const int c = tid + i * NT;
-      if (c < E / 4) buf[i] = ldp<Q>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo));
+      if (c < E / 4) buf[i] = ldp<Q>(a + base + tile_offset<T, M>(tq[i], w, lomask, geo));
     }
   }
   for (; tile < geo.tiles; tile += gridDim.x) {
@@ -884,7 +901,7 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
 #pragma unroll
       for (int i = 0; i < QPT; ++i) {
         const int c = tid + i * NT;
-        if (c < E / 4) buf[i] = ldp<Q>(a + nb + tile_offset<T, M>(4 * c, w, lomask, geo));
+        if (c < E / 4) buf[i] = ldp<Q>(a + nb + tile_offset<T, M>(tq[i], w, lomask, geo));
       }
     }
     __syncthreads();
@@ -892,7 +909,7 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
 #pragma unroll
     for (int i = 0; i < QPT; ++i) {
       const int c = tid + i * NT;
-      if (c < E / 4) stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), lds_get_quad(s, lb[i], le));
+      if (c < E / 4) stp(a + base + tile_offset<T, M>(tq[i], w, lomask, geo), lds_get_quad(s, lb[i], le));
     }
     __syncthreads();
   }
@@ -923,7 +940,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
     const int c = tid + i * NT;
     if (c < E / 4)
       lds_put_quad(s, LAYOUT ? lds_quad_base<M>(c, geo) : 4 * c, LAYOUT ? geo.lsh[0] : 0,
-                   ldp<Q>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo)));
+                   ldp<Q>(a + base + tile_offset<T, M>(LAYOUT ? quad_tile_index<M>(c, geo) : 4 * c, w, lomask, geo)));
   }
   __syncthreads();
   tile_ops<T, M, false, GEN2Q, NT>(s, ops, nops, prm);
@@ -931,7 +948,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
   for (int i = 0; i < QPT; ++i) {
     const int c = tid + i * NT;
     if (c < E / 4)
-      stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo),
+      stp(a + base + tile_offset<T, M>(LAYOUT ? quad_tile_index<M>(c, geo) : 4 * c, w, lomask, geo),
           lds_get_quad(s, LAYOUT ? lds_quad_base<M>(c, geo) : 4 * c, LAYOUT ? geo.lsh[0] : 0));
   }
 }
@@ -1930,6 +1947,13 @@ int parse_fused(const dm_state *h, const double *p, int64_t avail, int opi, Fuse
   for (int d = 0; d < m; ++d) {
     fp->geo.lsh[d] = 2 * lds_digit[d];
     fp->geo.ident = fp->geo.ident && lds_digit[d] == d;
+  }
+  // copy order: tile digits 1..m-1 by ascending LDS digit
+  {
+    int k = 0;
+    for (int l = 0; l < m; ++l)
+      for (int d = 1; d < m; ++d)
+        if (lds_digit[d] == l) fp->geo.qord[k++] = d;
   }
   for (TileOp &op : fp->ops) {
     const int nd = op.kind == TOP_1Q ? 1 : (op.kind == TOP_DENSE ? op.k : 2);
