@@ -96,7 +96,7 @@ def load():
         if _lib is not None:
             return _lib
         if not os.path.exists(_LIB_PATH):
-            raise NativeError('libdmsim.so not built (expected at %s); run __graft_entry__.build()' % _LIB_PATH)
+            raise NativeError('libdmsim.so not built (expected at %s); build it with dmsim._build.build()' % _LIB_PATH)
         lib = ctypes.CDLL(_LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)
