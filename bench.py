@@ -283,10 +283,11 @@ def batch_mode(args):
     lo, hi = (int(x) for x in args.batch_widths.split('-'))
     count = args.batch_circuits
     if hi > 7 and count > 12:
-        # Circuits above 7 qubits keep their state in HBM and run one kernel
-        # per operation: 512 circuits of 6-12 qubits enqueue ~4.8e5 launches
-        # per run.  A run of that batch ended the process on MI355X without
-        # an error message, and it stays off until the cause is known.
+        # Circuits above 7 qubits keep their state in HBM.  A run of 512
+        # circuits of 6-12 qubits (then one kernel per operation, ~4.8e5
+        # launches per run) ended the process on MI355X without an error
+        # message, and large HBM-tier batches stay off until the cause is
+        # known.
         print('batch config: %d circuits with widths up to %d refused (at most 12 when the batch holds circuits of '
               'more than 7 qubits)' % (count, hi), file=sys.stderr)
         sys.exit(3)
@@ -399,7 +400,7 @@ def main():
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-accuracy', action='store_true')
-    ap.add_argument('--fusion', default='auto', choices=['auto', 'greedy', 'learned'])
+    ap.add_argument('--fusion', default='auto', choices=['auto', 'greedy', 'search', 'learned'])
     ap.add_argument('--quick', action='store_true', help='time only the plain fp64 and headline paths')
     ap.add_argument('--config', default='headline', choices=['headline', 'batch', 'precision', 'thermal'])
     ap.add_argument('--layers', type=int, default=400, help='precision config: layers')

@@ -1921,28 +1921,56 @@ int parse_fused(const dm_state *h, const double *p, int64_t avail, int opi, Fuse
     at += nv;
     fp->ops.push_back(op);
   }
-  // LDS layout.  A two-qubit operation on tile digits 0-2 makes the lanes of
-  // a wave hit 4-16 times the same LDS bank (measured: a CX on digits 0 and 1
-  // costs 3.5x a CX on digits 3 and 4).  Then the digits of two-qubit
-  // operations go to the top LDS digits and the others below, in ascending
-  // order; single-qubit operations on low digits are cheap in place (the
-  // lowest digit has a vector path) and do not move digits, since a moved
-  // digit makes the HBM <-> LDS copy itself conflict.  Operation digits are
-  // rewritten to LDS digits.
-  bool used[8] = {false}, low2q = false, dense = false;
-  for (const TileOp &op : fp->ops) {
-    dense = dense || op.kind == TOP_DENSE;
-    if (op.kind != TOP_CX && op.kind != TOP_2Q) continue;
-    used[op.d[0]] = used[op.d[1]] = true;
-    low2q = low2q || op.d[0] < 3 || op.d[1] < 3;
+  // LDS layout.  The 64 lanes of a wave enumerate the lowest LDS digits an
+  // operation leaves free, and the 64 banks (4 bytes each) are LDS digits
+  // 0-2: a two-qubit operation with j of its digits among LDS digits 0-2
+  // hits each bank 4^j times (measured on MI355X: a CX on tile digits 0 and
+  // 1 costs 3.5x a CX on digits 3 and 4), a single-qubit operation on LDS
+  // digit 1 or 2 4 times (LDS digit 0 has a vector path).  The three tile
+  // digits placed at LDS digits 0-2 (in order; the others follow in
+  // ascending order) minimise the sum of these hit counts over the pass,
+  // single-qubit operations weighted 1/4 and any layout other than the
+  // identity charged 2 (its HBM <-> LDS copy conflicts or splits segments).
+  // Matrix-core passes keep the identity.  Operation digits are rewritten to
+  // LDS digits.
+  bool dense = false;
+  for (const TileOp &op : fp->ops) dense = dense || op.kind == TOP_DENSE;
+  int lds_digit[8];
+  for (int d = 0; d < 8; ++d) lds_digit[d] = d;
+  if (!dense && m >= 4) {
+    auto cost = [&](const int *bot) {
+      double c = (bot[0] == 0 && bot[1] == 1 && bot[2] == 2) ? 0.0 : 2.0;
+      auto in_bot = [&](int d) { return d == bot[0] || d == bot[1] || d == bot[2]; };
+      for (const TileOp &op : fp->ops) {
+        if (op.kind == TOP_CX || op.kind == TOP_2Q) {
+          const int j = (int)in_bot(op.d[0]) + (int)in_bot(op.d[1]);
+          c += j == 0 ? 1.0 : (j == 1 ? 4.0 : 16.0);
+        } else if (op.kind == TOP_1Q) {
+          c += 0.25 * (op.d[0] == bot[0] ? 1.0 : (in_bot(op.d[0]) ? 4.0 : 1.0));
+        }
+      }
+      return c;
+    };
+    int best[3] = {0, 1, 2};
+    double best_c = cost(best);
+    for (int a = 0; a < m; ++a)
+      for (int b = 0; b < m; ++b)
+        for (int c = 0; c < m; ++c) {
+          if (a == b || a == c || b == c) continue;
+          const int bot[3] = {a, b, c};
+          const double v = cost(bot);
+          if (v < best_c - 1e-9) {
+            best_c = v;
+            best[0] = a;
+            best[1] = b;
+            best[2] = c;
+          }
+        }
+    int nl = 0;
+    for (int k = 0; k < 3; ++k) lds_digit[best[k]] = nl++;
+    for (int d = 0; d < m; ++d)
+      if (d != best[0] && d != best[1] && d != best[2]) lds_digit[d] = nl++;
   }
-  if (dense || !low2q)  // matrix-core passes and high operations keep the identity layout
-    for (int d = 0; d < 8; ++d) used[d] = false;
-  int lds_digit[8], nl = 0;
-  for (int d = 0; d < m; ++d)
-    if (!used[d]) lds_digit[d] = nl++;
-  for (int d = 0; d < m; ++d)
-    if (used[d]) lds_digit[d] = nl++;
   fp->geo.ident = 1;
   for (int d = 0; d < m; ++d) {
     fp->geo.lsh[d] = 2 * lds_digit[d];
@@ -2027,7 +2055,9 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
   //    workgroups);
   //  - other runs of >= 4^3 with 2 or more resident workgroups per CU:
   //    k_tile_prog with one tile per workgroup, whose loads all issue before
-  //    the LDS writes;
+  //    the LDS writes; in FP32, where k_tile_prog's registers leave one
+  //    resident workgroup and k_tile_stream's leave two (7-digit tiles with
+  //    a moved LDS layout), k_tile_stream;
   //  - otherwise (runs of 4^2, whose neighbouring tiles share HBM lines, or
   //    one resident workgroup per CU): persistent workgroups that prefetch
   //    their next tile, 8 (resp. 4) per resident slot, with consecutive
@@ -2036,7 +2066,8 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
   do {                                                                                                           \
     const int per_cu = tile_resident_per_cu<T, MM, D, G, NTH, L>();                                             \
     const bool long_runs = fp.geo.w >= 3 && fp.geo.tiles <= (1ull << 30);                                     \
-    if (long_runs && !D && sizeof(T) == 4 && stream_resident_per_cu<T, MM, G, NTH, L>() * NTH / 64 >= 28) {    \
+    const int s_cu = D ? 0 : stream_resident_per_cu<T, MM, G, NTH, L>();                                        \
+    if (long_runs && !D && sizeof(T) == 4 && (s_cu * NTH / 64 >= 28 || (per_cu < 2 && s_cu >= 2))) {            \
       hipLaunchKernelGGL((k_tile_stream<T, MM, G, NTH, L>), dim3((unsigned)fp.geo.tiles), dim3(NTH), 0,          \
                          h->stream, a, fp.geo, d_ops, nops, d_vals);                                             \
       break;                                                                                                     \

@@ -5,11 +5,13 @@ after every gate: the gate, then memory noise on every qubit (one level per
 gate, dm_simulator.py:1172-1177), and finally an ensemble Z measurement of
 all qubits with polarisation retention ``depolarization_factor``.  Small
 registers (<= ``lds_max_qubits``) run whole inside one workgroup's LDS; the
-others keep their state in HBM and run on parallel streams; the batch is
-captured once into a hipGraph and replayed (``dm_batch_*`` in dmsim.h).
+others keep their state in HBM and run on parallel streams, as fused tile
+programs from 6 qubits up; the batch is captured once into a hipGraph and
+replayed (``dm_batch_*`` in dmsim.h).
 
 ``autotune`` picks the LDS cutoff width and the workgroup size by timing the
-candidates on the batch itself and caches the choice per (device, dtype).
+candidates on the batch itself and caches the choice per (precision, CU
+count, circuit count, width range).
 """
 import ctypes
 import itertools
@@ -23,8 +25,9 @@ from .program import Program
 
 _TUNE_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'batch_tuning.json')
 
-# circuits above the LDS cutoff run one kernel per operation; a graph-captured
-# run holds at most 16384 launches (dm_batch_run), about a dozen such circuits
+# circuits above the LDS cutoff keep their state in HBM; a graph-captured run
+# holds at most 16384 launches (dm_batch_run), and large HBM-tier batches stay
+# off until the failure of a 512-circuit run is understood
 MAX_HBM_CIRCUITS = 12
 
 

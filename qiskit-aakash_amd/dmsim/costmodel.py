@@ -8,8 +8,10 @@ from the pass's features:
   kind (one-hot: plain 1q, plain cx, tile sequence, tile dense), number of
   block qubits k, counts of 1- and 2-qubit gates in the pass, the length of
   the contiguous run the tile keeps in HBM (4^w coefficients), storage
-  precision, register size, lowest / highest block position, and a bitmask
-  of the block's positions (their strides 4^p).
+  precision, register size, lowest / highest block position, a bitmask of
+  the block's positions (their strides 4^p), the number of 2-qubit gates on
+  a qubit at the three lowest positions (they move the tile's LDS layout)
+  and the number of distinct qubits the 2-qubit gates use.
 
 The target is ln(time / state bytes), so one network covers register sizes.
 Training data are GPU timings of passes drawn from the candidate blocks of
@@ -27,7 +29,7 @@ import numpy as np
 
 from . import fusion, native, workloads
 
-N_FEATURES = 32
+N_FEATURES = 34
 HIDDEN = 64
 MAX_POS = 20
 TILE = 6
@@ -41,7 +43,17 @@ def tile_run_digits(n, positions):
     return fusion.run_digits(n, positions, fusion.tile_for(n, [n - 1 - p for p in positions]))
 
 
-def features(kind, n, qubits, n1, n2, precision):
+def gate_stats(n, gates):
+    """(n1, n2, n2low, d2) of a pass's gates: 1- and 2-qubit gate counts, the
+    2-qubit gates with a qubit at the three lowest positions, and the distinct
+    qubits of the 2-qubit gates."""
+    low = fusion.low_qubits(n)
+    two = [g for g in gates if len(g.qubits) == 2]
+    return (len(gates) - len(two), len(two), sum(1 for g in two if low.intersection(g.qubits)),
+            len({q for g in two for q in g.qubits}))
+
+
+def features(kind, n, qubits, n1, n2, precision, n2low=0, d2=0):
     """Feature vector (length N_FEATURES) of one pass."""
     pos = sorted(n - 1 - q for q in qubits)
     x = np.zeros(N_FEATURES, dtype=np.float32)
@@ -63,6 +75,8 @@ def features(kind, n, qubits, n1, n2, precision):
     for p in pos:
         if p < MAX_POS:
             x[12 + p] = 1.0
+    x[32] = math.log2(1 + n2low) / 4.0
+    x[33] = d2 / 7.0
     return x
 
 
@@ -100,29 +114,25 @@ class CostModel:
         np.savez(path, params=self.params, meta=json.dumps(self.meta))
 
     def predict_many(self, rows, n, precision):
-        """Seconds for each (kind, qubits, n1, n2) row."""
-        X = np.stack([features(k, n, q, a, b, precision) for k, q, a, b in rows])
+        """Seconds for each (kind, qubits, n1, n2[, n2low, d2]) row."""
+        X = np.stack([features(r[0], n, r[1], r[2], r[3], precision, *r[4:]) for r in rows])
         y = native.mlp_predict(self.params, N_FEATURES, HIDDEN, X)
         return np.exp(y.astype(np.float64)) * state_gib(n, precision)
 
-    def _time(self, kind, n, qs, n1, n2):
-        key = (kind, n, tuple(sorted(qs)), n1, n2, self.precision)
+    def _time(self, kind, n, qs, st):
+        key = (kind, n, tuple(sorted(qs))) + tuple(st) + (self.precision,)
         t = self._cache.get(key)
         if t is None:
-            t = float(self.predict_many([(kind, qs, n1, n2)], n, self.precision)[0])
+            t = float(self.predict_many([(kind, qs) + tuple(st)], n, self.precision)[0])
             self._cache[key] = t
         return t
 
-    def _counts(self, gates):
-        n2 = sum(1 for g in gates if len(g.qubits) == 2)
-        return len(gates) - n2, n2
-
     def best_mode(self, n, qs, gates):
-        n1, n2 = self._counts(gates)
+        st = gate_stats(n, gates)
         if len(qs) < 2 or n < TILE:
             return 'seq'
-        ts = self._time('seq', n, qs, n1, n2)
-        td = self._time('dense', n, qs, n1, n2) if len(qs) <= 4 else float('inf')
+        ts = self._time('seq', n, qs, st)
+        td = self._time('dense', n, qs, st) if len(qs) <= 4 else float('inf')
         return 'seq' if ts <= td else 'dense'
 
     def block_times(self, n, blocks):
@@ -130,33 +140,33 @@ class CostModel:
         prediction for the rows not cached yet)."""
         rows, keys = [], []
         for qs, gates in blocks:
-            n1, n2 = self._counts(gates)
+            st = gate_stats(n, gates)
             kind = pass_kind(qs, gates, 'seq')
             want = [kind] if kind != 'seq' else (['seq', 'dense'] if 2 <= len(qs) <= 4 else ['seq'])
             ks = []
             for kd in want:
-                key = (kd, n, tuple(sorted(qs)), n1, n2, self.precision)
+                key = (kd, n, tuple(sorted(qs))) + st + (self.precision,)
                 ks.append(key)
                 if key not in self._cache:
-                    rows.append((kd, qs, n1, n2))
+                    rows.append((kd, qs) + st)
             keys.append(ks)
         if rows:
             uniq = {}
             for r in rows:
-                uniq.setdefault((r[0], n, tuple(sorted(r[1])), r[2], r[3], self.precision), r)
+                uniq.setdefault((r[0], n, tuple(sorted(r[1]))) + tuple(r[2:]) + (self.precision,), r)
             t = self.predict_many(list(uniq.values()), n, self.precision)
             for key, tv in zip(uniq.keys(), t):
                 self._cache[key] = float(tv)
         return [min(self._cache[k] for k in ks) for ks in keys]
 
     def block_time(self, n, qs, gates):
-        n1, n2 = self._counts(gates)
+        st = gate_stats(n, gates)
         kind = pass_kind(qs, gates, 'seq')
         if kind != 'seq':
-            return self._time(kind, n, qs, n1, n2)
-        t = self._time('seq', n, qs, n1, n2)
+            return self._time(kind, n, qs, st)
+        t = self._time('seq', n, qs, st)
         if 2 <= len(qs) <= 4:
-            t = min(t, self._time('dense', n, qs, n1, n2))
+            t = min(t, self._time('dense', n, qs, st))
         return t
 
 
@@ -192,14 +202,13 @@ def candidate_passes(seeds, n, gates_per_circuit=200, kmax=4):
                 first += 1
             if first >= len(noisy):
                 break
-            options = fusion.candidate_blocks(noisy, done, first, kmax, n)
+            options = fusion.candidate_blocks(noisy, done, first, kmax, n, n >= fusion.MIN_TILE_QUBITS)
             for qs, mem in options:
                 gs = [noisy[i] for i in mem]
-                n2 = sum(1 for g in gs if len(g.qubits) == 2)
                 modes = ('seq', 'dense') if 2 <= len(qs) <= 4 and len(mem) > 1 else ('seq',)
                 for mode in modes:
                     kind = pass_kind(qs, gs, mode)
-                    key = (kind, tuple(sorted(qs)), len(gs) - n2, n2)
+                    key = (kind, tuple(sorted(qs))) + gate_stats(n, gs)
                     specs.setdefault(key, (kind, qs, gs))
             qs, mem = options[int(rng.integers(len(options)))]
             for i in mem:
@@ -254,11 +263,11 @@ def collect(n=16, seeds=range(1000, 3000), precisions=('fp32', 'fp64'), max_spec
         t0 = time.perf_counter()
         for i, (kind, qs, gs) in enumerate(specs):
             t = time_pass(st, kind, qs, gs)
-            n2 = sum(1 for g in gs if len(g.qubits) == 2)
-            X.append(features(kind, n, qs, len(gs) - n2, n2, prec))
+            n1, n2, n2low, d2 = gate_stats(n, gs)
+            X.append(features(kind, n, qs, n1, n2, prec, n2low, d2))
             y.append(math.log(t / state_gib(n, prec)))
-            rows.append({'kind': kind, 'qubits': list(qs), 'n1': len(gs) - n2, 'n2': n2, 'precision': prec,
-                         'n': n, 'seconds': t})
+            rows.append({'kind': kind, 'qubits': list(qs), 'n1': n1, 'n2': n2, 'n2low': n2low, 'd2': d2,
+                         'precision': prec, 'n': n, 'seconds': t})
             if log and (i + 1) % 200 == 0:
                 log('collected %d/%d %s passes (%.0f s)' % (i + 1, len(specs), prec, time.perf_counter() - t0))
         st.close()

@@ -18,6 +18,13 @@ LDS-staged tiles, or the block's dense PTM on the matrix cores.  ``plan`` choose
   single gate on its plain kernel is always a candidate;
 * ``search``  -- the same candidates, scored by gate count alone (every pass
   costed alike): the learned planner without its network.
+
+A sequential tile pass holds the block's qubits and the lowest other
+positions (its contiguous HBM runs).  With tile passes of the default size
+the learned planner also weighs, for each candidate, the block that takes
+every gate its tile holds (k_max then counts only the qubits outside the
+three lowest positions): more gates per pass, at the price of more LDS work
+per tile, which the network prices.
 """
 import collections
 import itertools
@@ -100,11 +107,12 @@ def block_ptm(gates, qubits):
     return np.ascontiguousarray(M.reshape(dim, dim))
 
 
-def _grow(gates, done, start, kmax, n):
-    """Largest block (<= kmax qubits) seeded at the first pending gate
-    ``start``.  Scan forward: a gate joins when none of its qubits is blocked
-    and the union stays within kmax; a gate that cannot join blocks all of its
-    qubits (every later gate on them must run after it)."""
+def _grow(gates, done, start, kmax, n, free=frozenset()):
+    """Largest block (<= kmax qubits outside ``free``) seeded at the first
+    pending gate ``start``.  Scan forward: a gate joins when none of its
+    qubits is blocked and the union stays within kmax; a gate that cannot
+    join blocks all of its qubits (every later gate on them must run after
+    it).  Returns the qubits counted against kmax and the members."""
     qs = []
     members = []
     blocked = set()
@@ -115,13 +123,14 @@ def _grow(gates, done, start, kmax, n):
         if any(q in blocked for q in gq):
             blocked.update(gq)
         else:
-            new = [q for q in gq if q not in qs]
+            new = [q for q in gq if q not in qs and q not in free]
             if len(qs) + len(new) <= kmax:
                 qs.extend(new)
                 members.append(i)
             else:
                 blocked.update(gq)
-        if qs and all(q in blocked for q in qs) and (len(qs) >= kmax or len(blocked.union(qs)) >= n):
+        if ((qs or free) and all(q in blocked for q in qs) and all(q in blocked for q in free)
+                and (len(qs) >= kmax or len(blocked.union(qs)) >= n)):
             break  # nothing further can join
     return qs, members
 
@@ -168,11 +177,46 @@ def _near_qubits(gates, done, start, seed, limit=6, window=48):
     return out[:limit]
 
 
-def candidate_blocks(gates, done, start, kmax, n):
-    """Distinct candidate blocks (qubits, members) seeded at ``start``: the
-    greedy growth for every k <= kmax, and every seed-containing set of at most
-    kmax qubits drawn from the nearby qubits (its members by ``_absorb``; its
-    qubits cut down to those its members use)."""
+def low_qubits(n):
+    """Qubits at the LOW_DIGITS lowest positions."""
+    return frozenset(range(max(0, n - LOW_DIGITS), n))
+
+
+def tile_qubits(n, T):
+    """Qubits of the tile a sequential pass over the qubits ``T`` (outside the
+    low positions) runs in: T, the low positions, and the next lowest
+    positions up to 6 digits (|T| <= 3) or 7 digits (|T| = 4).  Its runs are
+    at least 4^3 coefficients long."""
+    m = min(n, 6 if len(T) <= 3 else 7)
+    S = set(T) | low_qubits(n)
+    p = 0
+    while len(S) < m:
+        S.add(n - 1 - p)
+        p += 1
+    return S
+
+
+def used_qubits(gates, members):
+    qs = []
+    for i in members:
+        for q in gates[i].qubits:
+            if q not in qs:
+                qs.append(q)
+    return qs
+
+
+def candidate_blocks(gates, done, start, kmax, n, tiles=False):
+    """Distinct candidate blocks (qubits, members) seeded at ``start``.
+
+    The greedy growth for every k <= kmax, and every seed-containing set of
+    at most kmax qubits drawn from the nearby qubits (its members by
+    ``_absorb``).  With ``tiles`` (sequential tile passes, n >=
+    MIN_TILE_QUBITS) also the tile-wide blocks: for each set T of qubits
+    outside the low positions (the greedy growth counting only such qubits,
+    for every k, and the seed's qubits with nearby ones), every gate
+    ``_absorb`` admits into T's whole tile (``tile_qubits``).  A block's
+    qubits are those its members use; the single seed gate is always a
+    candidate."""
     seed = list(gates[start].qubits)
     cands = {}
 
@@ -180,12 +224,7 @@ def candidate_blocks(gates, done, start, kmax, n):
         key = tuple(members)
         if key in cands or not members:
             return
-        qs = []
-        for i in members:
-            for q in gates[i].qubits:
-                if q not in qs:
-                    qs.append(q)
-        cands[key] = (qs, members)
+        cands[key] = (used_qubits(gates, members), members)
 
     add([start])
     for k in range(2, kmax + 1):
@@ -194,20 +233,40 @@ def candidate_blocks(gates, done, start, kmax, n):
     for extra in range(1, kmax - len(seed) + 1):
         for T in itertools.combinations(near, extra):
             add(_absorb(gates, done, start, seed + list(T)))
+    if not tiles:
+        return list(cands.values())
+    low = low_qubits(n)
+    sets = set()
+    for k in range(1, kmax + 1):
+        sets.add(frozenset(_grow(gates, done, start, k, n, low)[0]))
+    hi = [q for q in seed if q not in low]
+    near = [q for q in _near_qubits(gates, done, start, seed, limit=6 + LOW_DIGITS) if q not in low][:6]
+    for extra in range(0, kmax - len(hi) + 1):
+        for X in itertools.combinations(near, extra):
+            sets.add(frozenset(hi) | frozenset(X))
+    for T in sorted(sets, key=sorted):
+        add(_absorb(gates, done, start, tile_qubits(n, T)))
     return list(cands.values())
 
 
 # blocks of 3-4 qubits run as tile passes of at least 6 digits (the MFMA form
 # needs k + column digits = 3+3 or 4+2), so smaller registers use k <= 2
 MIN_TILE_QUBITS = 6
+# every sequential tile pass holds the LOW_DIGITS lowest positions (its runs
+# of 4^3 contiguous coefficients), so gates on their qubits join a block
+# without widening its tile
+LOW_DIGITS = 3
 
 
-def plan(gates, n, kmax=4, method='greedy', cost=None):
-    """Blocks as (qubits, [gate indices]) in execution order."""
+def plan(gates, n, kmax=4, method='greedy', cost=None, tiles=True):
+    """Blocks as (qubits, [gate indices]) in execution order.  With ``tiles``
+    (blocks run as sequential tile passes of the default size) the learned
+    planner's candidates include the tile-wide blocks (``candidate_blocks``)."""
     done = [False] * len(gates)
     blocks = []
     first = 0
     allow_mfma = n >= MIN_TILE_QUBITS
+    tiles = tiles and allow_mfma and method == 'learned'
     spent = [0.0, 0]  # predicted seconds and gates of the blocks chosen so far
     while True:
         while first < len(gates) and done[first]:
@@ -219,7 +278,7 @@ def plan(gates, n, kmax=4, method='greedy', cost=None):
             qs, mem = _grow(gates, done, first, km, n)
         elif method in ('learned', 'search'):
             km = kmax if allow_mfma else min(kmax, 2)
-            cands = candidate_blocks(gates, done, first, km, n)
+            cands = candidate_blocks(gates, done, first, km, n, tiles)
             if method == 'learned':
                 # predicted net saving: the gates a block takes, valued at the
                 # plan's running cost per gate, minus its weighted own time
@@ -293,9 +352,10 @@ def tile_for(n, qs):
     """Tile size of a sequential pass over block ``qs``: 6 digits (32 KB of
     LDS in FP64) unless that leaves runs of only 4^2 coefficients, which
     stream at about two thirds of the rate of longer runs; then 7 digits
-    (512 threads, 128 KB in FP64) when the register allows it."""
+    (512 threads, 128 KB in FP64) when the register allows it.  A block of 7
+    qubits takes 7 digits."""
     pos = [n - 1 - q for q in qs]
-    if n >= 7 and run_digits(n, pos, 6) < 3 and run_digits(n, pos, 7) > run_digits(n, pos, 6):
+    if n >= 7 and (len(qs) > 6 or (run_digits(n, pos, 6) < 3 and run_digits(n, pos, 7) > run_digits(n, pos, 6))):
         return 7
     return 6
 
@@ -324,7 +384,7 @@ def add_block(prog, gates, qs, mem, mode='seq', tile=None):
         tile = tile_for(n, qs) if n >= 6 else 6
     if len(qs) == 1:
         prog.one_qubit(qs[0], block_ptm(sub, qs))
-    elif mode == 'seq' and n >= tile:
+    elif (mode == 'seq' or len(qs) > 4) and n >= tile:
         prog.fused(tile, qs, [_subop(g) for g in compose_single_qubit_runs(sub)])
     elif len(qs) == 2:
         # block_ptm's pair index is i_qs0 + 4 i_qs1: qs[1] is the leading qubit
@@ -349,8 +409,9 @@ def plain_program(gates, n):
 def fused_program(gates, n, kmax=4, method='greedy', cost=None, mode='seq', tile=None):
     """Program of fused passes; with a cost model each block also gets the
     cheaper of the sequential and dense forms.  ``tile`` fixes the tile size
-    of sequential passes (default: ``tile_for``)."""
-    blocks = plan(gates, n, kmax, method, cost)
+    of sequential passes (default: ``tile_for``); blocks take the gates of
+    their whole tile only with the default size in mode 'seq'."""
+    blocks = plan(gates, n, kmax, method, cost, tiles=(mode == 'seq' and tile is None))
     prog = Program(n)
     for qs, mem in blocks:
         m = mode

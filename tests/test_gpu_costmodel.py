@@ -30,8 +30,9 @@ def _seconds(prog, st, reps=2):
 
 def test_shipped_model_predicts_like_the_oracle():
     model = costmodel.default_model()
-    rows = [('seq', [0, 1, 2, 3], 5, 3), ('dense', [4, 5, 6], 4, 2), ('plain1q', [7], 1, 0), ('plaincx', [2, 3], 0, 1)]
-    X = np.stack([costmodel.features(k, 16, q, a, b, 'fp32') for k, q, a, b in rows])
+    rows = [('seq', [0, 1, 2, 3, 15], 5, 3, 1, 4), ('dense', [4, 5, 6], 4, 2, 0, 3), ('plain1q', [7], 1, 0, 0, 0),
+            ('plaincx', [2, 3], 0, 1, 0, 2)]
+    X = np.stack([costmodel.features(r[0], 16, r[1], r[2], r[3], 'fp32', *r[4:]) for r in rows])
     got = native.mlp_predict(model.params, costmodel.N_FEATURES, costmodel.HIDDEN, X)
     want, _ = mlp.forward(model.params.astype(np.float64), costmodel.N_FEATURES, costmodel.HIDDEN,
                           X.astype(np.float64))
@@ -49,10 +50,7 @@ def test_model_ranks_measured_pass_times():
     st = _state(n, 'fp32')
     measured = [costmodel.time_pass(st, kind, qs, gs, reps=2) for kind, qs, gs in pick]
     st.close()
-    rows = []
-    for kind, qs, gs in pick:
-        n2 = sum(1 for g in gs if len(g.qubits) == 2)
-        rows.append((kind, qs, len(gs) - n2, n2))
+    rows = [(kind, qs) + costmodel.gate_stats(n, gs) for kind, qs, gs in pick]
     pred = model.predict_many(rows, n, 'fp32')
     rank = lambda v: np.argsort(np.argsort(v))
     rho = np.corrcoef(rank(np.asarray(measured)), rank(pred))[0, 1]
