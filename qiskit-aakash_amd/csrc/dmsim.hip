@@ -607,7 +607,11 @@ __global__ __launch_bounds__(kBlock) void k_block_mfma(T *__restrict__ a, const 
 enum { TOP_1Q = 1, TOP_CX = 2, TOP_2Q = 3, TOP_DENSE = 4 };
 
 struct TileOp {
-  int32_t kind, k, d[4], param, pad;
+  int32_t kind, k, d[4], param;
+  // Swizzled passes: the order in which a thread's group index g enumerates
+  // the tile digits the operation leaves free (3 bits per digit, first digit
+  // in bits 0-2); the first ones cover distinct bank fields (lds_swz).
+  int32_t lanes;
 };
 
 struct TileGeom {
@@ -617,54 +621,38 @@ struct TileGeom {
   int nr;
   int rpos[kMaxQubits];
   uint64_t tiles;
-  // LDS layout: tile digit j sits at LDS digit lsh[j] / 2.  The digits the
-  // operations act on are placed above the others, so that the strided LDS
-  // accesses of an operation fall on distinct banks; ident marks the
-  // identity layout.
-  int lsh[8];
+  // ident: tile index i sits at LDS index i.  Otherwise the LDS index is
+  // lds_swz(i), and the operations enumerate their elements in the order of
+  // TileOp::lanes.
   int ident;
-  // Under a moved layout, quad c of the HBM <-> LDS copy is the tile index
-  // with digit qord[k] = base-4 digit k of c: consecutive lanes then vary the
-  // tile digits of the lowest LDS digits (distinct banks), at the price of
-  // shorter HBM segments per lane.
-  int qord[8];
 };
 
-// LDS index of tile element 4c (the quad of tile digit 0 starting there)
-// under the layout of geo; the quad's elements are 1 << geo.lsh[0] apart.
-template <int M>
-__device__ __forceinline__ int lds_quad_base(int c, const TileGeom &geo) {
-  if (geo.ident) return 4 * c;
-  int base = 0;
-#pragma unroll
-  for (int k = 0; k < M - 1; ++k) base |= ((c >> (2 * k)) & 3) << geo.lsh[geo.qord[k]];
-  return base;
+// Swizzled LDS index.  The 64 banks of 4 bytes are LDS index bits 0-5 (tile
+// digits 0-2); bits 2-3 and 4-5 take the XOR of tile digits 3 and 4, so the
+// bank fields are [d0, d1 ^ d3, d2 ^ d4].  An operation on digit 1 or 2
+// still has a free digit (3 or 4) that sweeps the field it occupies, and
+// the quads of tile digit 0 stay contiguous.  A bijection on [0, 4^M) for
+// M >= 5 that only permutes bits 2-5.
+template <bool SWZ>
+__device__ __forceinline__ int lds_swz(int i) {
+  return SWZ ? (i ^ (((i >> 6) & 15) << 2)) : i;
 }
-// Tile index (digit 0 zero) of quad c of the copy.
-template <int M>
-__device__ __forceinline__ int quad_tile_index(int c, const TileGeom &geo) {
-  if (geo.ident) return 4 * c;
-  int t = 0;
-#pragma unroll
-  for (int k = 0; k < M - 1; ++k) t |= ((c >> (2 * k)) & 3) << (2 * geo.qord[k]);
-  return t;
+
+// Tile index with the base-4 digits of g placed at the tile digits listed in
+// ord (3 bits each), the operation's own digits left zero.
+__device__ __forceinline__ int lane_deposit(int g, int ord, int nf) {
+  int b = 0;
+  for (int k = 0; k < nf; ++k) b |= ((g >> (2 * k)) & 3) << (2 * ((ord >> (3 * k)) & 7));
+  return b;
+}
+
+template <typename T>
+__device__ __forceinline__ void lds_put_quad(T *s, int lb, const Pack<T, 4> &q) {
+  *reinterpret_cast<Pack<T, 4> *>(&s[lb]) = q;
 }
 template <typename T>
-__device__ __forceinline__ void lds_put_quad(T *s, int lb, int le, const Pack<T, 4> &q) {
-  if (le == 0) {
-    *reinterpret_cast<Pack<T, 4> *>(&s[lb]) = q;
-  } else {
-#pragma unroll
-    for (int e = 0; e < 4; ++e) s[lb + (e << le)] = q.v[e];
-  }
-}
-template <typename T>
-__device__ __forceinline__ Pack<T, 4> lds_get_quad(const T *s, int lb, int le) {
-  if (le == 0) return *reinterpret_cast<const Pack<T, 4> *>(&s[lb]);
-  Pack<T, 4> q;
-#pragma unroll
-  for (int e = 0; e < 4; ++e) q.v[e] = s[lb + (e << le)];
-  return q;
+__device__ __forceinline__ Pack<T, 4> lds_get_quad(const T *s, int lb) {
+  return *reinterpret_cast<const Pack<T, 4> *>(&s[lb]);
 }
 
 __device__ __forceinline__ int ins_zero(int g, int j) {
@@ -735,9 +723,37 @@ __device__ __forceinline__ uint64_t tile_offset(int t, int w, int lomask, const 
   return off;
 }
 
+// A two-qubit operation on the 16 components v[c][t] (c: digit op.d[0],
+// t: digit op.d[1]) of one pair group, in place: the CX signed permutation
+// with its optional per-pair scale, or (GEN2Q) a 16x16 PTM whose first row
+// is the identity's.
+template <typename T, bool GEN2Q>
+__device__ __forceinline__ void tile_2q(T (&v)[4][4], const TileOp &op, const T *__restrict__ pv) {
+  if (!GEN2Q || op.kind == TOP_CX) {
+    Scale16<T> sv;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) sv.s[q] = op.k ? pv[q] : T(1);
+    cx_map<T, T, true, true>(v, sv);
+  } else {
+    T w[16];
+#pragma unroll
+    for (int r = 1; r < 16; ++r) {
+      T acc = T(0);
+#pragma unroll
+      for (int c = 0; c < 16; ++c) acc += pv[16 * r + c] * v[c >> 2][c & 3];
+      w[r] = acc;
+    }
+#pragma unroll
+    for (int r = 1; r < 16; ++r) v[r >> 2][r & 3] = w[r];
+  }
+}
+
 // The operations of a tile program on the LDS tile s (all threads of the
-// workgroup take part; a barrier follows each operation).
-template <typename T, int M, bool DENSE, bool GEN2Q, int NT>
+// workgroup take part; a barrier follows each operation).  SWZ: the tile is
+// stored at lds_swz of each tile index, and strided operations enumerate
+// their element groups in the digit order of TileOp::lanes, so that the 64
+// lanes of a wave fall on distinct banks.
+template <typename T, int M, bool DENSE, bool GEN2Q, int NT, bool SWZ>
 __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, int nops, const T *__restrict__ prm) {
   constexpr int E = 1 << (2 * M);
   constexpr int G1 = (E / 4 + NT - 1) / NT;   // 1q groups per thread
@@ -766,7 +782,7 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
 #pragma unroll
           for (int q = 0; q < GC; ++q) {
             const int g = tid + (q0 + q) * NT;
-            if (g < E / 4) x[q] = *reinterpret_cast<const Q *>(&s[4 * g]);
+            if (g < E / 4) x[q] = *reinterpret_cast<const Q *>(&s[lds_swz<SWZ>(4 * g)]);
           }
 #pragma unroll
           for (int q = 0; q < GC; ++q) {
@@ -777,11 +793,14 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
 #pragma unroll
             for (int c = 0; c < 3; ++c)
               y.v[c + 1] = r[c][0] * x[q].v[0] + r[c][1] * x[q].v[1] + r[c][2] * x[q].v[2] + r[c][3] * x[q].v[3];
-            *reinterpret_cast<Q *>(&s[4 * g]) = y;
+            *reinterpret_cast<Q *>(&s[lds_swz<SWZ>(4 * g)]) = y;
           }
         }
       } else {
         const int st = 1 << (2 * j);
+        int ko[4];  // LDS offsets of the four components (XOR masks when swizzled)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ko[c] = lds_swz<SWZ>(c * st);
 #pragma unroll
         for (int q0 = 0; q0 < G1; q0 += GC) {
           T x[GC][4];
@@ -789,10 +808,10 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
 #pragma unroll
           for (int q = 0; q < GC; ++q) {
             const int g = tid + (q0 + q) * NT;
-            bb[q] = ins_zero(g, j);
+            bb[q] = SWZ ? lds_swz<true>(lane_deposit(g, op.lanes, M - 1)) : ins_zero(g, j);
             if (g < E / 4) {
 #pragma unroll
-              for (int c = 0; c < 4; ++c) x[q][c] = s[bb[q] + c * st];
+              for (int c = 0; c < 4; ++c) x[q][c] = s[SWZ ? (bb[q] ^ ko[c]) : (bb[q] + ko[c])];
             }
           }
 #pragma unroll
@@ -801,7 +820,7 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
             if (g >= E / 4) continue;
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-              s[bb[q] + (c + 1) * st] =
+              s[SWZ ? (bb[q] ^ ko[c + 1]) : (bb[q] + ko[c + 1])] =
                   r[c][0] * x[q][0] + r[c][1] * x[q][1] + r[c][2] * x[q][2] + r[c][3] * x[q][3];
           }
         }
@@ -809,31 +828,74 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
     } else if (op.kind == TOP_CX || (GEN2Q && op.kind == TOP_2Q)) {
       const int jc = op.d[0], jt = op.d[1];
       const int jl = min(jc, jt), jh = max(jc, jt);
-      const int sc = 1 << (2 * jc), stt = 1 << (2 * jt);
-      for (int g = tid; g < E / 16; g += NT) {
-        const int b = ins_zero(ins_zero(g, jl), jh);
-        T v[4][4];
+      if (SWZ && jl == 0) {
+        // one digit is tile digit 0: the four quads along the other digit
+        // are one vector access each
+        int ko[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
+        for (int u = 0; u < 4; ++u) ko[u] = lds_swz<true>(u << (2 * jh));
+        for (int g = tid; g < E / 16; g += NT) {
+          const int b = lds_swz<true>(lane_deposit(g, op.lanes, M - 2));
+          Q qd[4];
 #pragma unroll
-          for (int t = 0; t < 4; ++t) v[c][t] = s[b + c * sc + t * stt];
-        if (!GEN2Q || op.kind == TOP_CX) {
-          Scale16<T> sv;
+          for (int u = 0; u < 4; ++u) qd[u] = *reinterpret_cast<const Q *>(&s[b ^ ko[u]]);
+          T v[4][4];
+          if (jc == 0) {
 #pragma unroll
-          for (int q = 0; q < 16; ++q) sv.s[q] = op.k ? pv[q] : T(1);
-          cx_map<T, T, true, true>(v, sv);
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+              for (int t = 0; t < 4; ++t) v[c][t] = qd[t].v[c];
+          } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+              for (int t = 0; t < 4; ++t) v[c][t] = qd[c].v[t];
+          }
+          tile_2q<T, GEN2Q>(v, op, pv);
+          if (jc == 0) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+              for (int t = 0; t < 4; ++t) qd[t].v[c] = v[c][t];
+          } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+              for (int t = 0; t < 4; ++t) qd[c].v[t] = v[c][t];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) *reinterpret_cast<Q *>(&s[b ^ ko[u]]) = qd[u];
+        }
+      } else {
+        int kc[4], kt[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          kc[u] = lds_swz<SWZ>(u << (2 * jc));
+          kt[u] = lds_swz<SWZ>(u << (2 * jt));
+        }
+        for (int g = tid; g < E / 16; g += NT) {
+          const int b = SWZ ? lds_swz<true>(lane_deposit(g, op.lanes, M - 2)) : ins_zero(ins_zero(g, jl), jh);
+          T v[4][4];
 #pragma unroll
           for (int c = 0; c < 4; ++c)
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
-              if (c | t) s[b + c * sc + t * stt] = v[c][t];
-        } else {
+            for (int t = 0; t < 4; ++t) v[c][t] = s[SWZ ? (b ^ kc[c] ^ kt[t]) : (b + kc[c] + kt[t])];
+          if (!GEN2Q || op.kind == TOP_CX) {
+            tile_2q<T, false>(v, op, pv);
 #pragma unroll
-          for (int r = 1; r < 16; ++r) {
-            T acc = T(0);
+            for (int c = 0; c < 4; ++c)
 #pragma unroll
-            for (int c = 0; c < 16; ++c) acc += pv[16 * r + c] * v[c >> 2][c & 3];
-            s[b + (r >> 2) * sc + (r & 3) * stt] = acc;
+              for (int t = 0; t < 4; ++t)
+                if (c | t) s[SWZ ? (b ^ kc[c] ^ kt[t]) : (b + kc[c] + kt[t])] = v[c][t];
+          } else {
+            // 16x16 PTM: each output row straight to LDS (no second buffer)
+#pragma unroll
+            for (int r = 1; r < 16; ++r) {
+              T acc = T(0);
+#pragma unroll
+              for (int c = 0; c < 16; ++c) acc += pv[16 * r + c] * v[c >> 2][c & 3];
+              s[SWZ ? (b ^ kc[r >> 2] ^ kt[r & 3]) : (b + kc[r >> 2] + kt[r & 3])] = acc;
+            }
           }
         }
       }
@@ -868,15 +930,8 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
     }
     return base;
   };
-  // quads of 4 consecutive tile indices are 4 consecutive coefficients; their
-  // LDS places are the same for every tile
-  int lb[QPT];
-#pragma unroll
-  for (int i = 0; i < QPT; ++i) lb[i] = LAYOUT ? lds_quad_base<M>(tid + i * NT, geo) : 4 * (tid + i * NT);
-  int tq[QPT];
-#pragma unroll
-  for (int i = 0; i < QPT; ++i) tq[i] = LAYOUT ? quad_tile_index<M>(tid + i * NT, geo) : 4 * (tid + i * NT);
-  const int le = LAYOUT ? geo.lsh[0] : 0;
+  // quad c = tid + i * NT is tile indices 4c..4c+3, 4 consecutive
+  // coefficients in HBM and (swizzled or not) in LDS
   Q buf[QPT];
   uint64_t tile = lid;
   if (tile < geo.tiles) {
@@ -884,7 +939,7 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
 #pragma unroll
     for (int i = 0; i < QPT; ++i) {
       const int c = tid + i * NT;
-      if (c < E / 4) buf[i] = ldp<Q>(a + base + tile_offset<T, M>(tq[i], w, lomask, geo));
+      if (c < E / 4) buf[i] = ldp<Q>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo));
     }
   }
   for (; tile < geo.tiles; tile += gridDim.x) {
@@ -892,7 +947,7 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
 #pragma unroll
     for (int i = 0; i < QPT; ++i) {
       const int c = tid + i * NT;
-      if (c < E / 4) lds_put_quad(s, lb[i], le, buf[i]);
+      if (c < E / 4) lds_put_quad(s, lds_swz<LAYOUT>(4 * c), buf[i]);
     }
     // prefetch the next tile while this one is computed
     const uint64_t next = tile + gridDim.x;
@@ -901,15 +956,15 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
 #pragma unroll
       for (int i = 0; i < QPT; ++i) {
         const int c = tid + i * NT;
-        if (c < E / 4) buf[i] = ldp<Q>(a + nb + tile_offset<T, M>(tq[i], w, lomask, geo));
+        if (c < E / 4) buf[i] = ldp<Q>(a + nb + tile_offset<T, M>(4 * c, w, lomask, geo));
       }
     }
     __syncthreads();
-    tile_ops<T, M, DENSE, GEN2Q, NT>(s, ops, nops, prm);
+    tile_ops<T, M, DENSE, GEN2Q, NT, LAYOUT>(s, ops, nops, prm);
 #pragma unroll
     for (int i = 0; i < QPT; ++i) {
       const int c = tid + i * NT;
-      if (c < E / 4) stp(a + base + tile_offset<T, M>(tq[i], w, lomask, geo), lds_get_quad(s, lb[i], le));
+      if (c < E / 4) stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), lds_get_quad(s, lds_swz<LAYOUT>(4 * c)));
     }
     __syncthreads();
   }
@@ -919,7 +974,7 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
 // buffer, so fewer registers and more resident workgroups per CU, which
 // overlap each other's HBM and LDS phases.
 template <typename T, int M, bool GEN2Q, int NT, bool LAYOUT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 1))) void k_tile_stream(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (LAYOUT ? 7 : 8) : 1))) void k_tile_stream(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
                                                         int nops, const T *__restrict__ prm) {
   constexpr int E = 1 << (2 * M);
   constexpr int QPT = (E / 4 + NT - 1) / NT;
@@ -938,18 +993,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
 #pragma unroll
   for (int i = 0; i < QPT; ++i) {
     const int c = tid + i * NT;
-    if (c < E / 4)
-      lds_put_quad(s, LAYOUT ? lds_quad_base<M>(c, geo) : 4 * c, LAYOUT ? geo.lsh[0] : 0,
-                   ldp<Q>(a + base + tile_offset<T, M>(LAYOUT ? quad_tile_index<M>(c, geo) : 4 * c, w, lomask, geo)));
+    if (c < E / 4) lds_put_quad(s, lds_swz<LAYOUT>(4 * c), ldp<Q>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo)));
   }
   __syncthreads();
-  tile_ops<T, M, false, GEN2Q, NT>(s, ops, nops, prm);
+  tile_ops<T, M, false, GEN2Q, NT, LAYOUT>(s, ops, nops, prm);
 #pragma unroll
   for (int i = 0; i < QPT; ++i) {
     const int c = tid + i * NT;
-    if (c < E / 4)
-      stp(a + base + tile_offset<T, M>(LAYOUT ? quad_tile_index<M>(c, geo) : 4 * c, w, lomask, geo),
-          lds_get_quad(s, LAYOUT ? lds_quad_base<M>(c, geo) : 4 * c, LAYOUT ? geo.lsh[0] : 0));
+    if (c < E / 4) stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), lds_get_quad(s, lds_swz<LAYOUT>(4 * c)));
   }
 }
 
@@ -1921,87 +1972,48 @@ int parse_fused(const dm_state *h, const double *p, int64_t avail, int opi, Fuse
     at += nv;
     fp->ops.push_back(op);
   }
-  // LDS layout.  The 64 lanes of a wave enumerate the lowest LDS digits an
-  // operation leaves free, and the 64 banks (4 bytes each) are LDS digits
-  // 0-2: a two-qubit operation with j of its digits among LDS digits 0-2
-  // hits each bank 4^j times (measured on MI355X: a CX on tile digits 0 and
-  // 1 costs 3.5x a CX on digits 3 and 4), a single-qubit operation on LDS
-  // digit 1 or 2 4 times (LDS digit 0 has a vector path).  The three tile
-  // digits placed at LDS digits 0-2 (in order; the others follow in
-  // ascending order) minimise the sum of these hit counts over the pass,
-  // single-qubit operations weighted 1/4 and any layout other than the
-  // identity charged 2 (its HBM <-> LDS copy conflicts or splits segments).
-  // Matrix-core passes keep the identity.  Operation digits are rewritten to
-  // LDS digits.
-  bool dense = false;
-  for (const TileOp &op : fp->ops) dense = dense || op.kind == TOP_DENSE;
-  int lds_digit[8];
-  for (int d = 0; d < 8; ++d) lds_digit[d] = d;
-  if (!dense && m >= 4) {
-    auto cost = [&](const int *bot) {
-      double c = (bot[0] == 0 && bot[1] == 1 && bot[2] == 2) ? 0.0 : 2.0;
-      auto in_bot = [&](int d) { return d == bot[0] || d == bot[1] || d == bot[2]; };
-      for (const TileOp &op : fp->ops) {
-        if (op.kind == TOP_CX || op.kind == TOP_2Q) {
-          const int j = (int)in_bot(op.d[0]) + (int)in_bot(op.d[1]);
-          c += j == 0 ? 1.0 : (j == 1 ? 4.0 : 16.0);
-        } else if (op.kind == TOP_1Q) {
-          c += 0.25 * (op.d[0] == bot[0] ? 1.0 : (in_bot(op.d[0]) ? 4.0 : 1.0));
+  // LDS layout.  The 64 banks (4 bytes each) are LDS index bits 0-5, tile
+  // digits 0-2.  Operations on digits 3 and up, and single-qubit operations
+  // on digit 0 (one vector access per group), fall on distinct banks in the
+  // identity layout.  A pass with a two-qubit operation on digits 0-2, or a
+  // single-qubit operation on digit 1 or 2, whose lanes would hit each bank
+  // 4 or 16 times (measured on MI355X: a CX on digits 0 and 1 costs 3.5x a
+  // CX on digits 3 and 4), stores the tile swizzled (lds_swz, bank fields
+  // [d0, d1^d3, d2^d4]) and gives each operation an order of its free digits
+  // whose first ones cover distinct fields: d0, then d1 (else d3), then d2
+  // (else d4), then the rest ascending.  A two-qubit operation on digit 0
+  // accesses whole quads; its first free digits cover fields 1 and 2.
+  // Matrix-core passes keep the identity layout.
+  bool dense = false, swz = false;
+  for (const TileOp &op : fp->ops) {
+    dense = dense || op.kind == TOP_DENSE;
+    if (op.kind == TOP_1Q) swz = swz || op.d[0] == 1 || op.d[0] == 2;
+    if (op.kind == TOP_CX || op.kind == TOP_2Q) swz = swz || op.d[0] < 3 || op.d[1] < 3;
+  }
+  swz = swz && !dense;
+  fp->geo.ident = swz ? 0 : 1;
+  if (swz)
+    for (TileOp &op : fp->ops) {
+      const int nd = op.kind == TOP_1Q ? 1 : 2;
+      bool busy[8] = {false};
+      for (int x = 0; x < nd; ++x) busy[op.d[x]] = true;
+      int ord[8], nf = 0;
+      auto take = [&](int d) {
+        if (d < m && !busy[d]) {
+          busy[d] = true;
+          ord[nf++] = d;
+          return true;
         }
-      }
-      return c;
-    };
-    int best[3] = {0, 1, 2};
-    double best_c = cost(best);
-    for (int a = 0; a < m; ++a)
-      for (int b = 0; b < m; ++b)
-        for (int c = 0; c < m; ++c) {
-          if (a == b || a == c || b == c) continue;
-          const int bot[3] = {a, b, c};
-          const double v = cost(bot);
-          if (v < best_c - 1e-9) {
-            best_c = v;
-            best[0] = a;
-            best[1] = b;
-            best[2] = c;
-          }
-        }
-    int nl = 0;
-    for (int k = 0; k < 3; ++k) lds_digit[best[k]] = nl++;
-    for (int d = 0; d < m; ++d)
-      if (d != best[0] && d != best[1] && d != best[2]) lds_digit[d] = nl++;
-  }
-  fp->geo.ident = 1;
-  for (int d = 0; d < m; ++d) {
-    fp->geo.lsh[d] = 2 * lds_digit[d];
-    fp->geo.ident = fp->geo.ident && lds_digit[d] == d;
-  }
-  // Copy order.  HBM order (quad digit k = tile digit k+1) keeps each lane's
-  // HBM segment long; LDS-digit order avoids bank conflicts.  HBM order is
-  // kept while the 64 lanes of a wave (tile digits 1-3) hit at most 4 times
-  // the same bank (64 banks of 4 bytes); measured on MI355X, a 16-way case
-  // went from 10.0 to 8.0 ms in LDS order and a 4-way one from 7.6 to 8.7.
-  {
-    const int es = (int)(h->dtype == DM_FP64 ? 2 : 1);  // banks per element
-    int hits[64] = {0}, worst = 0;
-    for (int lane = 0; lane < 64; ++lane) {
-      int addr = 0;
-      for (int k = 0; k < 3 && k + 1 < m; ++k) addr += ((lane >> (2 * k)) & 3) << fp->geo.lsh[k + 1];
-      worst = std::max(worst, ++hits[(addr * es) & 63]);
+        return false;
+      };
+      take(0);
+      if (!take(1)) take(3);
+      if (!take(2)) take(4);
+      for (int d = 0; d < m; ++d) take(d);
+      int packed = 0;
+      for (int k = 0; k < nf; ++k) packed |= ord[k] << (3 * k);
+      op.lanes = packed;
     }
-    int k = 0;
-    if (worst <= 4) {
-      for (int d = 1; d < m; ++d) fp->geo.qord[k++] = d;
-    } else {
-      for (int l = 0; l < m; ++l)
-        for (int d = 1; d < m; ++d)
-          if (lds_digit[d] == l) fp->geo.qord[k++] = d;
-    }
-  }
-  for (TileOp &op : fp->ops) {
-    const int nd = op.kind == TOP_1Q ? 1 : (op.kind == TOP_DENSE ? op.k : 2);
-    for (int d = 0; d < nd; ++d) op.d[d] = lds_digit[op.d[d]];
-  }
   return DM_OK;
 }
 
