@@ -2141,25 +2141,31 @@ int prepare_program(dm_state *h, const dm_op *ops, int nops, const double *param
   pp->offs.assign(nops, 0);
   pp->pos.assign(4 * (size_t)nops, 0);
   pp->top_off.assign(pp->fused.size(), 0);
+  // Tile operations are gathered for every fused pass, also when no pass
+  // has parameter values (noiseless CX only): the kernels read them from
+  // d_ops.
   std::vector<TileOp> tops;
-  if (pool) {
-    std::vector<T> staging(pool);
-    size_t off = 0;
-    for (int i = 0; i < nops; ++i) {
-      if (ops[i].kind == DM_OP_BLOCK) {
-        prep_block_matrix<T>(h->n, ops[i].qubits, ops[i].k, params + ops[i].param, staging.data() + off,
-                             &pp->pos[4 * (size_t)i]);
-        pp->offs[i] = off;
-        off += (size_t)1 << (4 * ops[i].k);
-      } else if (ops[i].kind == DM_OP_FUSED) {
-        FusedPass &fp = pp->fused[pp->fused_of[i]];
-        pp->offs[i] = off;
-        for (double v : fp.vals) staging[off++] = (T)v;
-        pp->top_off[pp->fused_of[i]] = tops.size();
-        tops.insert(tops.end(), fp.ops.begin(), fp.ops.end());
-      }
+  std::vector<T> staging(pool);
+  size_t off = 0;
+  for (int i = 0; i < nops; ++i) {
+    if (ops[i].kind == DM_OP_BLOCK) {
+      prep_block_matrix<T>(h->n, ops[i].qubits, ops[i].k, params + ops[i].param, staging.data() + off,
+                           &pp->pos[4 * (size_t)i]);
+      pp->offs[i] = off;
+      off += (size_t)1 << (4 * ops[i].k);
+    } else if (ops[i].kind == DM_OP_FUSED) {
+      FusedPass &fp = pp->fused[pp->fused_of[i]];
+      pp->offs[i] = off;
+      for (double v : fp.vals) staging[off++] = (T)v;
+      pp->top_off[pp->fused_of[i]] = tops.size();
+      tops.insert(tops.end(), fp.ops.begin(), fp.ops.end());
     }
-    if (int rc = ensure(&h->d_mat, &h->mat_cap, pool * sizeof(T))) return rc;
+  }
+  if (pool || !tops.empty()) {
+    // a valid (if unread) value pointer for passes without values
+    if (int rc = ensure(&h->d_mat, &h->mat_cap, std::max<size_t>(pool, 1) * sizeof(T))) return rc;
+  }
+  if (pool) {
     HIP_TRY(hipStreamSynchronize(h->stream));  // earlier kernels may still read d_mat / d_ops
     HIP_TRY(hipMemcpy(h->d_mat, staging.data(), pool * sizeof(T), hipMemcpyHostToDevice));
   }

@@ -309,6 +309,37 @@ def test_fused_tile_sizes_match_plain(prec, tile):
 
 
 @pytest.mark.parametrize('prec', PRECS)
+def test_fused_passes_without_parameter_values(prec):
+    """Passes of noiseless CX operations carry no parameter values; their
+    tile operations still reach the GPU, on a fresh handle and on one whose
+    previous program left other operations behind."""
+    from dmsim import fusion, workloads
+    n = 8
+    gates = [g for g in fusion.noisy_gates(workloads.local_u3_cx(n, 160, 5), 0.0) if g.kind == 'cx']
+    assert all(g.R is None for g in gates)
+    rng = np.random.default_rng(3)
+    a = rand_state(n, rng)
+    plain = fusion.plain_program(gates, n)
+    fused, blocks = fusion.fused_program(gates, n, 4, 'greedy', None, 'seq')
+    assert any(len(mem) > 1 for _, mem in blocks)
+    s1 = make(n, 'fp64', a)
+    plain.run(s1)
+    want = s1.download()
+    s1.close()
+    t = 1e-14 * 2.0 ** -n * 50 if prec == 'fp64' else 2e-6 * 2.0 ** -n * 50
+    s2 = make(n, prec, a)
+    fused.run(s2)
+    assert np.max(np.abs(s2.download() - want)) <= t
+    other, _ = fusion.fused_program(fusion.noisy_gates(workloads.local_u3_cx(n, 60, 6), 1e-3), n, 4, 'greedy',
+                                    None, 'seq')
+    other.run(s2)
+    s2.upload(a)
+    fused.run(s2)
+    assert np.max(np.abs(s2.download() - want)) <= t
+    s2.close()
+
+
+@pytest.mark.parametrize('prec', PRECS)
 def test_fused_passes_with_dense_two_qubit_ops(prec):
     """TSP CX gates are dense 16x16 operations inside the tile passes."""
     from dmsim import fusion, workloads
