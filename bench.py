@@ -33,7 +33,7 @@ for _p in (ROOT, os.path.join(ROOT, 'qiskit-aakash_amd')):
 
 import numpy as np  # noqa: E402
 
-from dmsim import fusion, native, workloads  # noqa: E402
+from dmsim import fusion, native, ptm, workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes
@@ -276,6 +276,41 @@ def dense_kernels_mode(args):
     return out
 
 
+def tile_ops_mode(args):
+    """Per-operation cost of tile passes at n qubits, FP32 (also a short run
+    for rocprofv3 LDS counter passes): for 6- and 7-digit tiles over the
+    highest qubits, one launch each of a pass with no operations, then with 8
+    copies of one operation: a 1q PTM on a high digit, on tile digit 0 and on
+    tile digit 1, a CX on two high digits, on digits 0 and 1 and on digits
+    2 and high (the last three swizzled where they touch digits 0-2).
+    Returns the milliseconds of each pass (HIP events)."""
+    from dmsim.program import Program
+    n = args.qubits
+    R = ptm.rows3(ptm.depolarizing1(1e-3) @ ptm.gate_ptm('u3', [0.3, 0.2, 0.1]))
+    S = ptm.depolarizing_pair_scale(1e-3)
+    lo = [n - 1, n - 2, n - 3]   # qubits at positions 0, 1, 2
+    cases = [('empty', []), ('1q high', [(native.SUB_1Q, [0], R)]), ('1q d0', [(native.SUB_1Q, [lo[0]], R)]),
+             ('1q d1', [(native.SUB_1Q, [lo[1]], R)]), ('cx high', [(native.SUB_CX, [0, 1], S)]),
+             ('cx d0 d1', [(native.SUB_CX, [lo[0], lo[1]], S)]), ('cx d2 high', [(native.SUB_CX, [lo[2], 0], S)])]
+    st = native.DeviceState(n, 'fp32')
+    zero_state(st, n)
+    out = {'n': n, 'ms': {}}
+    for m, qs in ((6, [0, 1, 2]), (7, [0, 1, 2, 3])):
+        for label, sub in cases:
+            prog = Program(n)
+            prog.fused(m, qs, sub * 8)
+            prog.run(st)   # upload and warm-up
+            st.synchronize()
+            st.profile(True)
+            st.profile_reset()
+            prog.run(st)
+            st.synchronize()
+            out['ms']['m%d %s' % (m, label)] = round(sum(st.profile_read(f)[1] for f in native.FAMILIES), 3)
+            st.profile(False)
+    st.close()
+    return out
+
+
 def batch_mode(args):
     """Batched circuits (U3/CX 70/30, memory noise after every gate, Z
     ensemble at the end), FP64 and FP32, autotuned, hipGraph replay."""
@@ -406,6 +441,7 @@ def main():
     ap.add_argument('--layers', type=int, default=400, help='precision config: layers')
     ap.add_argument('--kernels', action='store_true', help='short per-kernel run (for rocprofv3 counters)')
     ap.add_argument('--kernels-dense', action='store_true', help='short matrix-core kernel run (MFMA counters)')
+    ap.add_argument('--tile-ops', action='store_true', help='per-operation cost of tile passes (LDS counters)')
     ap.add_argument('--batch-circuits', type=int, default=512, help='batch config: circuits')
     ap.add_argument('--batch-widths', default='6-12', help='batch config: qubit range lo-hi')
     args = ap.parse_args()
@@ -419,6 +455,9 @@ def main():
         return
     if args.kernels_dense:
         print(json.dumps(dense_kernels_mode(args)), flush=True)
+        return
+    if args.tile_ops:
+        print(json.dumps(tile_ops_mode(args)), flush=True)
         return
     if args.config != 'headline':
         fn = {'batch': batch_mode, 'precision': precision_mode, 'thermal': thermal_mode}[args.config]
