@@ -906,7 +906,13 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
   }
 }
 
-template <typename T, int M, bool DENSE, bool GEN2Q, int NT, bool LAYOUT>
+// A fused pass: each workgroup loads a tile (the M tile digits of geo) into
+// LDS, applies the pass's operations and writes it back, looping over tiles
+// with stride gridDim.x and prefetching its next tile into registers.  The
+// host launches grid = tiles (remap = 0) or a persistent grid (remap = 1,
+// consecutive tiles kept on one XCD).  SWZ: the tile sits in LDS at lds_swz
+// of each index.
+template <typename T, int M, bool DENSE, bool GEN2Q, int NT, bool SWZ>
 __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
                                                       int nops, const T *__restrict__ prm, int remap) {
   constexpr int E = 1 << (2 * M);
@@ -947,7 +953,7 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
 #pragma unroll
     for (int i = 0; i < QPT; ++i) {
       const int c = tid + i * NT;
-      if (c < E / 4) lds_put_quad(s, lds_swz<LAYOUT>(4 * c), buf[i]);
+      if (c < E / 4) lds_put_quad(s, lds_swz<SWZ>(4 * c), buf[i]);
     }
     // prefetch the next tile while this one is computed
     const uint64_t next = tile + gridDim.x;
@@ -960,11 +966,11 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
       }
     }
     __syncthreads();
-    tile_ops<T, M, DENSE, GEN2Q, NT, LAYOUT>(s, ops, nops, prm);
+    tile_ops<T, M, DENSE, GEN2Q, NT, SWZ>(s, ops, nops, prm);
 #pragma unroll
     for (int i = 0; i < QPT; ++i) {
       const int c = tid + i * NT;
-      if (c < E / 4) stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), lds_get_quad(s, lds_swz<LAYOUT>(4 * c)));
+      if (c < E / 4) stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), lds_get_quad(s, lds_swz<SWZ>(4 * c)));
     }
     __syncthreads();
   }
@@ -972,9 +978,11 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
 
 // One tile per workgroup (grid = tiles, tile = block index): no prefetch
 // buffer, so fewer registers and more resident workgroups per CU, which
-// overlap each other's HBM and LDS phases.
-template <typename T, int M, bool GEN2Q, int NT, bool LAYOUT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (LAYOUT ? 7 : 8) : 1))) void k_tile_stream(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
+// overlap each other's HBM and LDS phases.  FP32: 8 waves per SIMD (64
+// VGPRs), 7 for swizzled tiles (72 VGPRs: their XOR addresses cannot use
+// instruction offsets; 64 spilled).
+template <typename T, int M, bool GEN2Q, int NT, bool SWZ>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (SWZ ? 7 : 8) : 1))) void k_tile_stream(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
                                                         int nops, const T *__restrict__ prm) {
   constexpr int E = 1 << (2 * M);
   constexpr int QPT = (E / 4 + NT - 1) / NT;
@@ -993,14 +1001,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
 #pragma unroll
   for (int i = 0; i < QPT; ++i) {
     const int c = tid + i * NT;
-    if (c < E / 4) lds_put_quad(s, lds_swz<LAYOUT>(4 * c), ldp<Q>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo)));
+    if (c < E / 4) lds_put_quad(s, lds_swz<SWZ>(4 * c), ldp<Q>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo)));
   }
   __syncthreads();
-  tile_ops<T, M, false, GEN2Q, NT, LAYOUT>(s, ops, nops, prm);
+  tile_ops<T, M, false, GEN2Q, NT, SWZ>(s, ops, nops, prm);
 #pragma unroll
   for (int i = 0; i < QPT; ++i) {
     const int c = tid + i * NT;
-    if (c < E / 4) stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), lds_get_quad(s, lds_swz<LAYOUT>(4 * c)));
+    if (c < E / 4) stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), lds_get_quad(s, lds_swz<SWZ>(4 * c)));
   }
 }
 
@@ -2068,8 +2076,7 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
   //  - other runs of >= 4^3 with 2 or more resident workgroups per CU:
   //    k_tile_prog with one tile per workgroup, whose loads all issue before
   //    the LDS writes; in FP32, where k_tile_prog's registers leave one
-  //    resident workgroup and k_tile_stream's leave two (7-digit tiles with
-  //    a moved LDS layout), k_tile_stream;
+  //    resident workgroup and k_tile_stream's leave two, k_tile_stream;
   //  - otherwise (runs of 4^2, whose neighbouring tiles share HBM lines, or
   //    one resident workgroup per CU): persistent workgroups that prefetch
   //    their next tile, 8 (resp. 4) per resident slot, with consecutive
