@@ -359,10 +359,10 @@ def precision_mode(args):
     n = 17
     gates = workloads.matching_layers(n, args.layers, args.seed)
     noisy = fusion.noisy_gates(gates, 5e-4)
-    prog, blocks = fusion.fused_program(noisy, n, 4, 'greedy', None, 'seq')
-    res = {}
+    prog, blocks, method, _ = headline_plan(noisy, n, args.fusion)
+    res = {'fusion': method}
     states = {}
-    log('precision config: %d gates, %d passes' % (len(noisy), len(prog)))
+    log('precision config: %d gates, %d passes (%s)' % (len(noisy), len(prog), method))
     for prec in ('fp64', 'fp32'):
         st = native.DeviceState(n, prec)
         zero_state(st, n)
@@ -402,7 +402,7 @@ def thermal_mode(args):
     qc = workloads.to_circuit(n, gates, barrier_after_each=True, measure=('partial', [0, 1, 2, 3], 'Z'))
     be = BasicAer.get_backend('dm_simulator')
     out = {}
-    for fusion_mode in ('none', 'greedy'):
+    for fusion_mode in ('none', 'greedy', 'learned'):
         t0 = time.perf_counter()
         r = execute(qc, be, decay_factor=g, decoherence_factor=f, thermal_factor=1.0,
                     compute_densitymatrix=False, fusion=fusion_mode).result()
@@ -415,14 +415,16 @@ def thermal_mode(args):
         log('%s: %s' % (fusion_mode, json.dumps({k: v for k, v in out[fusion_mode].items()
                                                   if k != 'partial_probability'})))
     pp = out['none']['partial_probability']
-    pq = out['greedy']['partial_probability']
     return {'metric': '14-qubit thermal brickwork+QFT, gates/s (FP64, execute)',
             'value': len(gates) / out['none']['running_time_taken'], 'unit': 'gates/s', 'n_gpus': 1,
             'higher_is_better': True, 'dtype': 'f64', 'data': 'synthetic',
             'config': {'workload': '14q, 30 brickwork layers + QFT, T1=50us T2=70us dt=50ns, barrier after every '
                                    'gate, partial Z on q0-3', 'plain': out['none'], 'fused': out['greedy'],
+                       'fused_learned': out['learned'],
                        'fused_gates_per_s': len(gates) / out['greedy']['running_time_taken'],
-                       'max_partial_prob_diff_fused_vs_plain': max(abs(pp[k] - pq[k]) for k in pp)}}
+                       'fused_learned_gates_per_s': len(gates) / out['learned']['running_time_taken'],
+                       'max_partial_prob_diff_fused_vs_plain': max(
+                           abs(pp[k] - out[m]['partial_probability'][k]) for m in ('greedy', 'learned') for k in pp)}}
 
 
 def main():

@@ -439,6 +439,7 @@ def plan_levels(levels, count, n, backend):
     if method == 'learned':
         from . import costmodel
         cost = costmodel.default_model()
+        cost.precision = backend._precision
     out = {}
     clock = 0
     while clock < count:
