@@ -107,48 +107,66 @@ def block_ptm(gates, qubits):
     return np.ascontiguousarray(M.reshape(dim, dim))
 
 
-def _grow(gates, done, start, kmax, n, free=frozenset()):
+def qubit_masks(gates):
+    """Bitmask of each gate's qubits (the planners' scans test masks)."""
+    return [sum(1 << q for q in g.qubits) for g in gates]
+
+
+def _grow(gates, done, start, kmax, n, free=frozenset(), masks=None):
     """Largest block (<= kmax qubits outside ``free``) seeded at the first
     pending gate ``start``.  Scan forward: a gate joins when none of its
     qubits is blocked and the union stays within kmax; a gate that cannot
     join blocks all of its qubits (every later gate on them must run after
-    it).  Returns the qubits counted against kmax and the members."""
+    it).  Returns the qubits counted against kmax and the members; with
+    ``free`` the scan stops as soon as the counted qubits are final, so only
+    they are complete (the caller recollects the members)."""
+    if masks is None:
+        masks = qubit_masks(gates)
+    fm = sum(1 << q for q in free)
+    full = (1 << n) - 1
+    qm = 0
     qs = []
     members = []
-    blocked = set()
+    blocked = 0
     for i in range(start, len(gates)):
         if done[i]:
             continue
-        gq = gates[i].qubits
-        if any(q in blocked for q in gq):
-            blocked.update(gq)
+        gm = masks[i]
+        if gm & blocked:
+            blocked |= gm
         else:
-            new = [q for q in gq if q not in qs and q not in free]
-            if len(qs) + len(new) <= kmax:
-                qs.extend(new)
+            new = gm & ~qm & ~fm
+            if len(qs) + new.bit_count() <= kmax:
+                if new:
+                    qs.extend(q for q in gates[i].qubits if (new >> q) & 1)
+                    qm |= new
                 members.append(i)
             else:
-                blocked.update(gq)
-        if ((qs or free) and all(q in blocked for q in qs) and all(q in blocked for q in free)
-                and (len(qs) >= kmax or len(blocked.union(qs)) >= n)):
+                blocked |= gm
+        if fm:
+            if len(qs) >= kmax or (blocked | qm | fm) == full:
+                break  # no further counted qubit can join
+        elif qm and not (qm & ~blocked) and (len(qs) >= kmax or (blocked | qm) == full):
             break  # nothing further can join
     return qs, members
 
 
-def _absorb(gates, done, start, qset):
+def _absorb(gates, done, start, qset, masks=None):
     """Gates of the block on qubit set ``qset`` seeded at the first pending
     gate ``start``: scanning forward, a gate joins when its qubits lie in the
     set and none is blocked; any other gate blocks all of its qubits."""
-    S = set(qset)
+    if masks is None:
+        masks = qubit_masks(gates)
+    S = sum(1 << q for q in qset)
     members = []
-    blocked = set()
+    blocked = 0
     for i in range(start, len(gates)):
         if done[i]:
             continue
-        gq = gates[i].qubits
-        if any(q in blocked for q in gq) or not S.issuperset(gq):
-            blocked.update(gq)
-            if S <= blocked:
+        gm = masks[i]
+        if (gm & blocked) or (gm & ~S):
+            blocked |= gm
+            if not (S & ~blocked):
                 break
         else:
             members.append(i)
@@ -205,7 +223,7 @@ def used_qubits(gates, members):
     return qs
 
 
-def candidate_blocks(gates, done, start, kmax, n, tiles=False):
+def candidate_blocks(gates, done, start, kmax, n, tiles=False, masks=None):
     """Distinct candidate blocks (qubits, members) seeded at ``start``.
 
     The greedy growth for every k <= kmax, and every seed-containing set of
@@ -217,6 +235,8 @@ def candidate_blocks(gates, done, start, kmax, n, tiles=False):
     ``_absorb`` admits into T's whole tile (``tile_qubits``).  A block's
     qubits are those its members use; the single seed gate is always a
     candidate."""
+    if masks is None:
+        masks = qubit_masks(gates)
     seed = list(gates[start].qubits)
     cands = {}
 
@@ -228,24 +248,24 @@ def candidate_blocks(gates, done, start, kmax, n, tiles=False):
 
     add([start])
     for k in range(2, kmax + 1):
-        add(_grow(gates, done, start, k, n)[1])
+        add(_grow(gates, done, start, k, n, masks=masks)[1])
     near = _near_qubits(gates, done, start, seed)
     for extra in range(1, kmax - len(seed) + 1):
         for T in itertools.combinations(near, extra):
-            add(_absorb(gates, done, start, seed + list(T)))
+            add(_absorb(gates, done, start, seed + list(T), masks))
     if not tiles:
         return list(cands.values())
     low = low_qubits(n)
     sets = set()
     for k in range(1, kmax + 1):
-        sets.add(frozenset(_grow(gates, done, start, k, n, low)[0]))
+        sets.add(frozenset(_grow(gates, done, start, k, n, low, masks)[0]))
     hi = [q for q in seed if q not in low]
     near = [q for q in _near_qubits(gates, done, start, seed, limit=6 + LOW_DIGITS) if q not in low][:6]
     for extra in range(0, kmax - len(hi) + 1):
         for X in itertools.combinations(near, extra):
             sets.add(frozenset(hi) | frozenset(X))
     for T in sorted(sets, key=sorted):
-        add(_absorb(gates, done, start, tile_qubits(n, T)))
+        add(_absorb(gates, done, start, tile_qubits(n, T), masks))
     return list(cands.values())
 
 
@@ -267,6 +287,7 @@ def plan(gates, n, kmax=4, method='greedy', cost=None, tiles=True):
     first = 0
     allow_mfma = n >= MIN_TILE_QUBITS
     tiles = tiles and allow_mfma and method == 'learned'
+    masks = qubit_masks(gates)
     spent = [0.0, 0]  # predicted seconds and gates of the blocks chosen so far
     while True:
         while first < len(gates) and done[first]:
@@ -275,10 +296,10 @@ def plan(gates, n, kmax=4, method='greedy', cost=None, tiles=True):
             break
         if method == 'greedy':
             km = kmax if allow_mfma else min(kmax, 2)
-            qs, mem = _grow(gates, done, first, km, n)
+            qs, mem = _grow(gates, done, first, km, n, masks=masks)
         elif method in ('learned', 'search'):
             km = kmax if allow_mfma else min(kmax, 2)
-            cands = candidate_blocks(gates, done, first, km, n, tiles)
+            cands = candidate_blocks(gates, done, first, km, n, tiles, masks)
             if method == 'learned':
                 # predicted net saving: the gates a block takes, valued at the
                 # plan's running cost per gate, minus its weighted own time
