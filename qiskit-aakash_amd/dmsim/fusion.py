@@ -37,13 +37,19 @@ from .program import Program
 NoisyGate = collections.namedtuple('NoisyGate', 'qubits R kind')
 
 # largest block each planner forms (qubits): 4, a Pauli transfer matrix of at
-# most 256 x 256
+# most 256 x 256 (the learned planner's tile-wide blocks count only the
+# qubits outside the LOW_DIGITS lowest positions)
 KMAX = {'greedy': 4, 'learned': 4, 'search': 4}
+
+# every sequential tile pass holds the LOW_DIGITS lowest positions (its runs
+# of 4^3 contiguous coefficients), so gates on their qubits join a block
+# without widening its tile
+LOW_DIGITS = 3
 
 # weight of a block's predicted time against the gates it takes, both in
 # seconds (gates at the plan's running cost per gate).  Measured on MI355X
-# (16 qubits, FP32, five held-out circuits): weights 1.0 and 0.5 plan 3 % and
-# 1.5 % slower than 0.25 and 0.1, which match the gate-count search.
+# (16 qubits, FP32, three held-out circuits, tile-wide candidates): weights
+# 0.25 and 0.5 plan alike, 1.0 is 4 % slower.
 TIME_WEIGHT = 0.25
 # kind: 'u3' (4x4 R), 'cx' (R = per-pair scale16 or None, signed permutation),
 #       '2q' (16x16 R)
@@ -215,6 +221,7 @@ def tile_qubits(n, T):
 
 
 def used_qubits(gates, members):
+    """Distinct qubits of the member gates, in order of first use."""
     qs = []
     for i in members:
         for q in gates[i].qubits:
@@ -272,10 +279,6 @@ def candidate_blocks(gates, done, start, kmax, n, tiles=False, masks=None):
 # blocks of 3-4 qubits run as tile passes of at least 6 digits (the MFMA form
 # needs k + column digits = 3+3 or 4+2), so smaller registers use k <= 2
 MIN_TILE_QUBITS = 6
-# every sequential tile pass holds the LOW_DIGITS lowest positions (its runs
-# of 4^3 contiguous coefficients), so gates on their qubits join a block
-# without widening its tile
-LOW_DIGITS = 3
 
 
 def plan(gates, n, kmax=4, method='greedy', cost=None, tiles=True):
