@@ -286,10 +286,11 @@ def candidate_blocks(gates, done, start, kmax, n, tiles=False, masks=None):
 MIN_TILE_QUBITS = 6
 
 
-def plan(gates, n, kmax=4, method='greedy', cost=None, tiles=True):
+def plan(gates, n, kmax=4, method='greedy', cost=None, tiles=True, lookahead=LOOKAHEAD):
     """Blocks as (qubits, [gate indices]) in execution order.  With ``tiles``
     (blocks run as sequential tile passes of the default size) the learned
-    planner's candidates include the tile-wide blocks (``candidate_blocks``)."""
+    planner's candidates include the tile-wide blocks (``candidate_blocks``);
+    ``lookahead`` <= 1 makes it choose without looking ahead."""
     done = [False] * len(gates)
     blocks = []
     first = 0
@@ -326,9 +327,9 @@ def plan(gates, n, kmax=4, method='greedy', cost=None, tiles=True):
                 score = [gc * per_gate - TIME_WEIGHT * tc for gc, tc in zip(g, t)]
                 # one step of lookahead: each of the LOOKAHEAD best candidates
                 # is credited with the best saving available right after it
-                top = sorted(range(len(cands)), key=lambda i: -score[i])[:LOOKAHEAD]
+                top = sorted(range(len(cands)), key=lambda i: -score[i])[:max(1, lookahead)]
                 total, nexts = {}, {}
-                for i in top:
+                for i in (top if lookahead > 1 else ()):
                     d2 = list(done)
                     for j in cands[i][1]:
                         d2[j] = True
@@ -342,7 +343,7 @@ def plan(gates, n, kmax=4, method='greedy', cost=None, tiles=True):
                         nxt = max(len(m2) * per_gate - TIME_WEIGHT * tt for (_, m2), tt in zip(c2, t2))
                         nexts[i] = (f2, c2)
                     total[i] = score[i] + nxt
-                pick = max(top, key=lambda i: total[i])
+                pick = max(top, key=lambda i: total.get(i, score[i]))
                 ahead = nexts.get(pick)
                 spent[0] += t[pick]
                 spent[1] += g[pick]
@@ -460,12 +461,12 @@ def plain_program(gates, n):
     return prog
 
 
-def fused_program(gates, n, kmax=4, method='greedy', cost=None, mode='seq', tile=None):
+def fused_program(gates, n, kmax=4, method='greedy', cost=None, mode='seq', tile=None, lookahead=LOOKAHEAD):
     """Program of fused passes; with a cost model each block also gets the
     cheaper of the sequential and dense forms.  ``tile`` fixes the tile size
     of sequential passes (default: ``tile_for``); blocks take the gates of
     their whole tile only with the default size in mode 'seq'."""
-    blocks = plan(gates, n, kmax, method, cost, tiles=(mode == 'seq' and tile is None))
+    blocks = plan(gates, n, kmax, method, cost, tiles=(mode == 'seq' and tile is None), lookahead=lookahead)
     prog = Program(n)
     for qs, mem in blocks:
         m = mode
@@ -510,6 +511,8 @@ def plan_levels(levels, count, n, backend):
             if noisy_levels:
                 gates += [NoisyGate((q,), mem_R, 'u3') for q in range(n)]
             clock += 1
-        prog, _ = fused_program(gates, n, KMAX[method], method, cost)
+        # below 15 qubits a pass takes about a millisecond and the lookahead's
+        # host time (about 2x the plain choice) costs more than it saves
+        prog, _ = fused_program(gates, n, KMAX[method], method, cost, lookahead=LOOKAHEAD if n >= 15 else 1)
         out[start] = (clock, prog.run)
     return out

@@ -50,6 +50,13 @@ def test_plans_are_valid(n, method, cpu_model):
     check_plan(gates, n, blocks)
 
 
+@pytest.mark.parametrize('lookahead', [1, 3])
+def test_learned_plans_with_and_without_lookahead_are_valid(lookahead, cpu_model):
+    n = 12
+    gates = fusion.noisy_gates(workloads.local_u3_cx(n, 300, 11), 1e-3)
+    check_plan(gates, n, fusion.plan(gates, n, 4, 'learned', cpu_model, lookahead=lookahead))
+
+
 def test_tile_wide_candidates():
     """With tiles the candidates include the classic ones; every candidate
     is a valid block of at most 7 qubits, the three lowest positions free."""
