@@ -15,8 +15,9 @@ LDS-staged tiles, or the block's dense PTM on the matrix cores.  ``plan`` choose
   by their predicted net saving -- their gates valued at the plan's running
   cost per gate, minus TIME_WEIGHT times their predicted time -- the
   predictions coming from the MLP cost model (dmsim.costmodel), and take, of
-  the LOOKAHEAD best, the one whose saving plus the best saving after it is
-  largest; a single gate on its plain kernel is always a candidate;
+  the best few, the one whose saving plus the best saving of the blocks
+  that can follow it (LOOKAHEAD) is largest; a single gate on its plain
+  kernel is always a candidate;
 * ``search``  -- the same candidates, scored by gate count alone (every pass
   costed alike): the learned planner without its network.
 
@@ -52,10 +53,12 @@ LOW_DIGITS = 3
 # (16 qubits, FP32, three held-out circuits, tile-wide candidates): weights
 # 0.25 and 0.5 plan alike, 1.0 is 4 % slower.
 TIME_WEIGHT = 0.25
-# candidates the learned planner scores one step ahead (five held-out
-# 400-gate circuits at 16 qubits: 178 instead of 187 passes, 3.4 % less
-# predicted time; 4 candidates gain another 0.2 %)
-LOOKAHEAD = 3
+# lookahead widths of the learned planner: its 3 best candidates are each
+# credited with the best saving of the two blocks that can follow (the 2
+# best next, then the best after each).  Five held-out 400-gate circuits at
+# 16 qubits: 173 passes and 5.1 % less predicted time than choosing without
+# lookahead (187), against 178 and 3.4 % for one block of lookahead (3,).
+LOOKAHEAD = (3, 2)
 # kind: 'u3' (4x4 R), 'cx' (R = per-pair scale16 or None, signed permutation),
 #       '2q' (16x16 R)
 
@@ -290,7 +293,7 @@ def plan(gates, n, kmax=4, method='greedy', cost=None, tiles=True, lookahead=LOO
     """Blocks as (qubits, [gate indices]) in execution order.  With ``tiles``
     (blocks run as sequential tile passes of the default size) the learned
     planner's candidates include the tile-wide blocks (``candidate_blocks``);
-    ``lookahead`` <= 1 makes it choose without looking ahead."""
+    ``lookahead`` gives its search widths (LOOKAHEAD), () none."""
     done = [False] * len(gates)
     blocks = []
     first = 0
@@ -325,26 +328,44 @@ def plan(gates, n, kmax=4, method='greedy', cost=None, tiles=True, lookahead=LOO
                     j = int(np.argmax(g))
                     per_gate = t[j] / g[j]
                 score = [gc * per_gate - TIME_WEIGHT * tc for gc, tc in zip(g, t)]
-                # one step of lookahead: each of the LOOKAHEAD best candidates
-                # is credited with the best saving available right after it
-                top = sorted(range(len(cands)), key=lambda i: -score[i])[:max(1, lookahead)]
-                total, nexts = {}, {}
-                for i in (top if lookahead > 1 else ()):
-                    d2 = list(done)
-                    for j in cands[i][1]:
+                # lookahead: each of the lookahead[0] best candidates is
+                # credited with the best total saving of the blocks that can
+                # follow it, searched over the lookahead[1:] best at each
+                # further step (the last step takes its best alone)
+                widths = tuple(lookahead)
+                top = sorted(range(len(cands)), key=lambda i: -score[i])[:max(1, widths[0] if widths else 1)]
+                seen = {}  # candidates of states reached by the lookahead
+
+                def after(members, base, f):
+                    """done list and first pending gate after ``members``."""
+                    d2 = list(base)
+                    for j in members:
                         d2[j] = True
-                    f2 = first
-                    while f2 < len(gates) and d2[f2]:
-                        f2 += 1
-                    nxt = 0.0
-                    if f2 < len(gates):
+                    while f < len(gates) and d2[f]:
+                        f += 1
+                    return d2, f
+
+                def best_after(d2, f2, depth, key):
+                    if f2 >= len(gates):
+                        return 0.0
+                    if key not in seen:
                         c2 = candidate_blocks(gates, d2, f2, km, n, tiles, masks)
                         t2 = cost.block_times(n, [(q2, [gates[j] for j in m2]) for q2, m2 in c2])
-                        nxt = max(len(m2) * per_gate - TIME_WEIGHT * tt for (_, m2), tt in zip(c2, t2))
-                        nexts[i] = (f2, c2)
-                    total[i] = score[i] + nxt
+                        seen[key] = (f2, c2, [len(m2) * per_gate - TIME_WEIGHT * tt for (_, m2), tt in zip(c2, t2)])
+                    _, c2, s2 = seen[key]
+                    if depth >= len(widths):
+                        return max(s2)
+                    order = sorted(range(len(c2)), key=lambda i: -s2[i])[:widths[depth]]
+                    return max(s2[i] + best_after(*after(c2[i][1], d2, f2), depth + 1, key + tuple(c2[i][1]))
+                               for i in order)
+
+                total = {}
+                if len(widths) >= 2 or (widths and widths[0] > 1):
+                    for i in top:
+                        total[i] = score[i] + best_after(*after(cands[i][1], done, first), 1, tuple(cands[i][1]))
                 pick = max(top, key=lambda i: total.get(i, score[i]))
-                ahead = nexts.get(pick)
+                nxt = seen.get(tuple(cands[pick][1]))
+                ahead = (nxt[0], nxt[1]) if nxt is not None else None
                 spent[0] += t[pick]
                 spent[1] += g[pick]
             else:
@@ -513,6 +534,6 @@ def plan_levels(levels, count, n, backend):
             clock += 1
         # below 15 qubits a pass takes about a millisecond and the lookahead's
         # host time (about 2x the plain choice) costs more than it saves
-        prog, _ = fused_program(gates, n, KMAX[method], method, cost, lookahead=LOOKAHEAD if n >= 15 else 1)
+        prog, _ = fused_program(gates, n, KMAX[method], method, cost, lookahead=LOOKAHEAD if n >= 15 else ())
         out[start] = (clock, prog.run)
     return out

@@ -50,7 +50,7 @@ def test_plans_are_valid(n, method, cpu_model):
     check_plan(gates, n, blocks)
 
 
-@pytest.mark.parametrize('lookahead', [1, 3])
+@pytest.mark.parametrize('lookahead', [(), (3,), (3, 2)])
 def test_learned_plans_with_and_without_lookahead_are_valid(lookahead, cpu_model):
     n = 12
     gates = fusion.noisy_gates(workloads.local_u3_cx(n, 300, 11), 1e-3)
