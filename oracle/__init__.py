@@ -2,9 +2,9 @@
 
 TEST INFRASTRUCTURE ONLY.  Nothing in the product package imports this
 directory; only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` use it, and only as the checker (or, in
-``bench.py``, as the timed CPU baseline), never as the thing measured or
-shipped.
+``cpu_baseline`` leg of ``bench.py`` use it, and only to check GPU results
+(or, in ``bench.py``, as the timed CPU baseline), never as the thing measured
+or shipped.
 
 Contents
 --------

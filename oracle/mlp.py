@@ -1,7 +1,7 @@
 """CPU restatement (FP64 numpy) of the fusion cost model's perceptron.
 
-TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``): the checker of the
-GPU forward / backward kernels in ``csrc/mlp.hip``.  No reference
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``): the reference the
+GPU forward / backward kernels in ``csrc/mlp.hip`` are tested against.  No reference
 counterpart exists; the oracle itself is pinned by central finite differences
 (tests/test_oracle_mlp.py).
 
