@@ -8,14 +8,17 @@ resident state.
 
 Paths measured:
   plain_fp64   one kernel per noisy gate, FP64 storage (the baseline);
-  fused_fp32   gates fused into k<=4-qubit Pauli-transfer blocks (MFMA), FP32
-               storage, partition chosen by the learned cost model when a
-               trained model is available, else by the greedy rule (the
-               chosen method is reported in config.fusion).
+  fused_fp32   gates fused into tile passes (one HBM round trip per block of
+               gates, applied in LDS), FP32 storage, partition chosen by the
+               learned cost model when a trained model is available, else by
+               the greedy rule (the chosen method is reported in
+               config.fusion).
 ``value`` is the fused FP32 throughput; the plain FP64 figure, the speed-up
 and the accuracy of the FP32 result against FP64 are reported beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+       python bench.py --config precision|thermal|batch
+       python bench.py --kernels | --kernels-dense | --tile-ops   (short runs for rocprofv3)
 """
 import argparse
 import json
