@@ -289,6 +289,52 @@ def candidate_blocks(gates, done, start, kmax, n, tiles=False, masks=None):
 MIN_TILE_QUBITS = 6
 
 
+def _state_after(gates, done, first, members):
+    """done list and first pending gate once ``members`` have run."""
+    d2 = list(done)
+    for j in members:
+        d2[j] = True
+    while first < len(gates) and d2[first]:
+        first += 1
+    return d2, first
+
+
+def _lookahead_pick(gates, done, first, cands, score, per_gate, widths, ctx):
+    """Index of the learned planner's choice among ``cands`` (scored by
+    ``score``), and the candidates of the state after it if the search built
+    them, as (first pending gate, candidates) or None.
+
+    Each of the widths[0] best candidates is credited with the best total
+    saving of the blocks that can follow it, searched over the widths[1:]
+    best at each further step; the step after the last width takes its best
+    alone.  With no widths (or a single width of 1) the best score wins."""
+    km, n, tiles, masks, cost = ctx
+    top = sorted(range(len(cands)), key=lambda i: -score[i])[:max(1, widths[0] if widths else 1)]
+    if not (len(widths) >= 2 or (widths and widths[0] > 1)):
+        return top[0], None
+    seen = {}  # path of member tuples -> (first pending gate, candidates, scores)
+
+    def best_after(d2, f2, depth, key):
+        if f2 >= len(gates):
+            return 0.0
+        if key not in seen:
+            c2 = candidate_blocks(gates, d2, f2, km, n, tiles, masks)
+            t2 = cost.block_times(n, [(q2, [gates[j] for j in m2]) for q2, m2 in c2])
+            seen[key] = (f2, c2, [len(m2) * per_gate - TIME_WEIGHT * tt for (_, m2), tt in zip(c2, t2)])
+        _, c2, s2 = seen[key]
+        if depth >= len(widths):
+            return max(s2)
+        order = sorted(range(len(c2)), key=lambda i: -s2[i])[:widths[depth]]
+        return max(s2[i] + best_after(*_state_after(gates, d2, f2, c2[i][1]), depth + 1, key + tuple(c2[i][1]))
+                   for i in order)
+
+    total = {i: score[i] + best_after(*_state_after(gates, done, first, cands[i][1]), 1, tuple(cands[i][1]))
+             for i in top}
+    pick = max(top, key=lambda i: total[i])
+    nxt = seen.get(tuple(cands[pick][1]))
+    return pick, ((nxt[0], nxt[1]) if nxt is not None else None)
+
+
 def plan(gates, n, kmax=4, method='greedy', cost=None, tiles=True, lookahead=LOOKAHEAD):
     """Blocks as (qubits, [gate indices]) in execution order.  With ``tiles``
     (blocks run as sequential tile passes of the default size) the learned
@@ -328,44 +374,8 @@ def plan(gates, n, kmax=4, method='greedy', cost=None, tiles=True, lookahead=LOO
                     j = int(np.argmax(g))
                     per_gate = t[j] / g[j]
                 score = [gc * per_gate - TIME_WEIGHT * tc for gc, tc in zip(g, t)]
-                # lookahead: each of the lookahead[0] best candidates is
-                # credited with the best total saving of the blocks that can
-                # follow it, searched over the lookahead[1:] best at each
-                # further step (the last step takes its best alone)
-                widths = tuple(lookahead)
-                top = sorted(range(len(cands)), key=lambda i: -score[i])[:max(1, widths[0] if widths else 1)]
-                seen = {}  # candidates of states reached by the lookahead
-
-                def after(members, base, f):
-                    """done list and first pending gate after ``members``."""
-                    d2 = list(base)
-                    for j in members:
-                        d2[j] = True
-                    while f < len(gates) and d2[f]:
-                        f += 1
-                    return d2, f
-
-                def best_after(d2, f2, depth, key):
-                    if f2 >= len(gates):
-                        return 0.0
-                    if key not in seen:
-                        c2 = candidate_blocks(gates, d2, f2, km, n, tiles, masks)
-                        t2 = cost.block_times(n, [(q2, [gates[j] for j in m2]) for q2, m2 in c2])
-                        seen[key] = (f2, c2, [len(m2) * per_gate - TIME_WEIGHT * tt for (_, m2), tt in zip(c2, t2)])
-                    _, c2, s2 = seen[key]
-                    if depth >= len(widths):
-                        return max(s2)
-                    order = sorted(range(len(c2)), key=lambda i: -s2[i])[:widths[depth]]
-                    return max(s2[i] + best_after(*after(c2[i][1], d2, f2), depth + 1, key + tuple(c2[i][1]))
-                               for i in order)
-
-                total = {}
-                if len(widths) >= 2 or (widths and widths[0] > 1):
-                    for i in top:
-                        total[i] = score[i] + best_after(*after(cands[i][1], done, first), 1, tuple(cands[i][1]))
-                pick = max(top, key=lambda i: total.get(i, score[i]))
-                nxt = seen.get(tuple(cands[pick][1]))
-                ahead = (nxt[0], nxt[1]) if nxt is not None else None
+                pick, ahead = _lookahead_pick(gates, done, first, cands, score, per_gate, tuple(lookahead),
+                                              (km, n, tiles, masks, cost))
                 spent[0] += t[pick]
                 spent[1] += g[pick]
             else:
