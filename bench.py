@@ -16,7 +16,14 @@ Paths measured:
 ``value`` is the fused FP32 throughput; the plain FP64 figure, the speed-up
 and the accuracy of the FP32 result against FP64 are reported beside it.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Timing: the fused FP32 program runs once (accuracy against FP64), then
+``warmup`` + ``steps`` times; the plain FP64 program, ~11 s per application,
+is timed on ``--plain-steps`` (default 1) applications, so the driver's
+``--steps 20 --warmup 5`` finishes in about a minute of GPU time.  The JSON
+line is printed before any optional work (``--all-paths``).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--plain-steps P] [--no-cpu-baseline]
+                       [--all-paths]
        python bench.py --config precision|thermal|batch
        python bench.py --kernels | --kernels-dense | --tile-ops   (short runs for rocprofv3)
 """
@@ -55,11 +62,15 @@ def zero_state(st, n):
     st.init_product(np.tile([1.0, 0.0, 0.0, 1.0], n), 0.5 ** n)
 
 
-def time_program(prog, n, precision, steps, warmup):
+def time_program(prog, n, precision, steps, warmup, st=None):
     """Wall time of `steps` program applications (after `warmup`), with HIP
-    event timing of every launch on the handle's stream."""
-    st = native.DeviceState(n, precision)
-    zero_state(st, n)
+    event timing of every launch on the handle's stream.  With ``st`` given
+    the applications continue on that state (left open); otherwise a fresh
+    |0..0> state is made and closed."""
+    own = st is None
+    if own:
+        st = native.DeviceState(n, precision)
+        zero_state(st, n)
     for _ in range(warmup):
         prog.run(st)
     st.synchronize()
@@ -72,7 +83,8 @@ def time_program(prog, n, precision, steps, warmup):
     t1 = time.perf_counter()
     fams = {f: st.profile_read(f) for f in native.FAMILIES}
     st.profile(False)
-    st.close()
+    if own:
+        st.close()
     return t1 - t0, fams
 
 
@@ -135,34 +147,34 @@ def streaming_fraction(fams, n, precision, counts):
     return (tot_b / tot_t / 1e9 / HBM_PEAK_GBS) if tot_t else None
 
 
-def accuracy(prog_fp32, prog_fp64, n):
-    """One application of each program from |0..0>: FP32 result vs FP64."""
-    a = native.DeviceState(n, 'fp64')
-    zero_state(a, n)
-    prog_fp64.run(a)
-    ref = a.download()
-    a.close()
-    b = native.DeviceState(n, 'fp32')
-    zero_state(b, n)
-    prog_fp32.run(b)
-    tr, pur = b.trace_purity()
-    marg = [b.probabilities([q], 'Z')[0] for q in range(n)]
-    fid = b.overlap(ref)
-    got = b.download()
-    b.close()
-    err = 0.0
-    chunk = 1 << 26
-    for i in range(0, ref.size, chunk):
-        err = max(err, float(np.max(np.abs(ref[i:i + chunk] - got[i:i + chunk]))))
-    ref_pur = float(2.0 ** n * np.dot(ref, ref))
-    ref_marg = []
-    for q in range(n):
-        # P(q=0) = (1 + 2^n a_{Z on q})/2
-        ref_marg.append(0.5 * (1.0 + ref[3 * 4 ** (n - 1 - q)] * 2.0 ** n))
-    return {'max_abs_coeff_err': err, 'max_abs_coeff_err_x2n': err * 2.0 ** n,
-            'fidelity_fp32_vs_fp64': fid / ref_pur, 'trace_fp32': tr, 'purity_fp32': pur, 'purity_fp64': ref_pur,
-            'max_marginal_err': float(np.max(np.abs(np.array(marg) - np.array(ref_marg)))),
-            'identity_coeff_exact': bool(got[0] == 2.0 ** -n)}
+def accuracy(st32, st64, n):
+    """FP32 state against the FP64 state of the same program applied to
+    |0..0>, compared on the device in FP64: max coefficient error, fidelity
+    (normalised overlap), trace and purity, and the n single-qubit marginals
+    P(q = 0)."""
+    cmp = st32.compare(st64)
+    tr, pur = st32.trace_purity()
+    m32 = np.array([st32.probabilities([q], 'Z')[0] for q in range(n)])
+    m64 = np.array([st64.probabilities([q], 'Z')[0] for q in range(n)])
+    a0 = float(st32.read([0])[0])
+    return {'max_abs_coeff_err': cmp['max_abs_diff'], 'max_abs_coeff_err_x2n': cmp['max_abs_diff'] * 2.0 ** n,
+            'fidelity_fp32_vs_fp64': cmp['dot'] / cmp['norm2_other'], 'trace_fp32': tr, 'purity_fp32': pur,
+            'purity_fp64': cmp['norm2_other'] * 2.0 ** n, 'max_marginal_err': float(np.max(np.abs(m32 - m64))),
+            'identity_coeff_exact': a0 == 2.0 ** -n}
+
+
+def host_cpu():
+    """(logical CPUs, CPU model) of this host."""
+    model = None
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count(), model
 
 
 def cpu_baseline(n_target, G, seed, budget_s=20.0, n_sample=13):
@@ -191,7 +203,9 @@ def cpu_baseline(n_target, G, seed, budget_s=20.0, n_sample=13):
             break
     el = time.perf_counter() - t0
     rate = done / el
+    ncpu, model = host_cpu()
     return {'value': rate / 4 ** (n_target - n), 'unit': 'gates/s', 'cores': 1, 'kind': 'port',
+            'host_logical_cpus': ncpu, 'cpu_model': model,
             'sample': '%d gates of the same generator at %d qubits (FP64 oracle, numpy, 1 thread) in %.1f s = '
                       '%.3f gates/s; scaled by 4^-%d to %d qubits' % (done, n, el, rate, n_target - n, n_target)}
 
@@ -430,7 +444,7 @@ def thermal_mode(args):
                            abs(pp[k] - out[m]['partial_probability'][k]) for m in ('greedy', 'learned') for k in pp)}}
 
 
-def main():
+def parse_args(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=2)
@@ -441,7 +455,9 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-accuracy', action='store_true')
     ap.add_argument('--fusion', default='auto', choices=['auto', 'greedy', 'search', 'learned'])
-    ap.add_argument('--quick', action='store_true', help='time only the plain fp64 and headline paths')
+    ap.add_argument('--plain-steps', type=int, default=1, help='timed applications of the plain FP64 program')
+    ap.add_argument('--all-paths', action='store_true',
+                    help='after the headline line, also time plain FP32, fused FP64 and the greedy forms (stderr)')
     ap.add_argument('--config', default='headline', choices=['headline', 'batch', 'precision', 'thermal'])
     ap.add_argument('--layers', type=int, default=400, help='precision config: layers')
     ap.add_argument('--kernels', action='store_true', help='short per-kernel run (for rocprofv3 counters)')
@@ -449,7 +465,11 @@ def main():
     ap.add_argument('--tile-ops', action='store_true', help='per-operation cost of tile passes (LDS counters)')
     ap.add_argument('--batch-circuits', type=int, default=512, help='batch config: circuits')
     ap.add_argument('--batch-widths', default='6-12', help='batch config: qubit range lo-hi')
-    args = ap.parse_args()
+    return ap.parse_args(argv)
+
+
+def main():
+    args = parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU: the path has no multi-GPU form in this build (--gpus %d refused)'
               % args.gpus, file=sys.stderr)
@@ -469,6 +489,26 @@ def main():
         print(json.dumps(fn(args)), flush=True)
         return
 
+    headline(args)
+
+
+def planned_applications(args):
+    """Whole-program applications of the 1000-gate workload each path makes
+    in a default headline run (the bound a CPU test checks): plain FP64 is
+    timed on ``--plain-steps`` applications after one warm-up over its first
+    PLAIN_WARMUP_GATES gates; the fused FP32 program runs once for the
+    accuracy comparison, then ``warmup`` + ``steps`` times."""
+    return {'plain_fp64': args.plain_steps + PLAIN_WARMUP_GATES / max(1, args.gates),
+            'fused_fp32': 1 + args.warmup + args.steps,
+            'other_paths': 5 * (1 + 2) if args.all_paths else 0}
+
+
+# gates of the plain FP64 program run once before its timed applications
+# (first launches of each kernel variant); one full application takes ~11 s
+PLAIN_WARMUP_GATES = 100
+
+
+def headline(args):
     n, G = args.qubits, args.gates
     gates = workloads.local_u3_cx(n, G, args.seed)
     noisy = fusion.noisy_gates(gates, 1e-3)
@@ -479,39 +519,45 @@ def main():
     log('plan: %s, %d passes for %d gates (%.2f s host); k histogram %s'
         % (method, len(fused), G, plan_s, np.bincount(ks, minlength=6).tolist()))
 
-    log('timing plain fp64 ...')
-    t_plain, f_plain = time_program(plain, n, 'fp64', args.steps, args.warmup)
-    log('  %.3f s for %d steps' % (t_plain, args.steps))
-    log('timing fused fp32 ...')
-    t_fused, f_fused = time_program(fused, n, 'fp32', args.steps, args.warmup)
-    log('  %.3f s for %d steps' % (t_fused, args.steps))
-    extra = {}
-    if not args.quick:
-        log('timing plain fp32, fused fp64, greedy dense-MFMA fp32 ...')
-        extra['plain_fp32'] = time_program(plain, n, 'fp32', args.steps, args.warmup)
-        extra['fused_fp64'] = time_program(fused, n, 'fp64', args.steps, args.warmup)
-        dense, _ = fusion.fused_program(noisy, n, 4, 'greedy', None, 'dense')
-        extra['greedy_dense_fp32'] = time_program(dense, n, 'fp32', args.steps, args.warmup)
-        dense3, _ = fusion.fused_program(noisy, n, 3, 'greedy', None, 'dense')
-        extra['greedy_dense_k3_fp32'] = time_program(dense3, n, 'fp32', args.steps, args.warmup)
-        if method != 'greedy':
-            gseq, _ = fusion.fused_program(noisy, n, 4, 'greedy', None, 'seq')
-            extra['greedy_seq_fp32'] = time_program(gseq, n, 'fp32', args.steps, args.warmup)
+    # plain FP64: a short warm-up, then --plain-steps whole applications from
+    # |0..0>; with one timed application the state is the FP64 reference the
+    # FP32 result is compared with
+    log('timing plain fp64 (%d application(s)) ...' % args.plain_steps)
+    st64 = native.DeviceState(n, 'fp64')
+    zero_state(st64, n)
+    fusion.plain_program(noisy[:PLAIN_WARMUP_GATES], n).run(st64)
+    zero_state(st64, n)
+    t_plain, f_plain = time_program(plain, n, 'fp64', args.plain_steps, 0, st=st64)
+    log('  %.3f s for %d application(s)' % (t_plain, args.plain_steps))
+    if args.plain_steps != 1 and not args.no_accuracy:
+        zero_state(st64, n)
+        plain.run(st64)
 
-    gps = lambda t: G * args.steps / t
-    value = gps(t_fused)
+    log('timing fused fp32 (%d warm-up, %d timed) ...' % (args.warmup, args.steps))
+    st32 = native.DeviceState(n, 'fp32')
+    zero_state(st32, n)
+    fused.run(st32)
+    acc = None
+    if not args.no_accuracy:
+        acc = accuracy(st32, st64, n)
+        log('accuracy: %s' % json.dumps(acc))
+    st64.close()
+    t_fused, f_fused = time_program(fused, n, 'fp32', args.steps, args.warmup, st=st32)
+    st32.close()
+    log('  %.3f s for %d steps' % (t_fused, args.steps))
+
     dom, roof, avg_s, byts, flops = kernel_roofline(f_fused, n, 'fp32')
     pdom, proof, _, _, _ = kernel_roofline(f_plain, n, 'fp64')
     if n == 16 and G == 1000 and args.seed == 0:
         roof['traffic'] = measured_traffic('%s_fp32' % dom, '%s/%d' % (method, len(fused)))
         proof['traffic'] = measured_traffic('plain_fp64_%s' % pdom, 'any')
-    acc = None if args.no_accuracy else accuracy(fused, plain, n)
-    if acc:
-        log('accuracy: %s' % json.dumps(acc))
     cpu = None if args.no_cpu_baseline else cpu_baseline(n, G, args.seed)
     if cpu:
         log('cpu baseline: %s' % cpu['sample'])
 
+    value = G * args.steps / t_fused
+    plain_gps = G * args.plain_steps / t_plain
+    step_s = t_fused / args.steps
     out = {
         'metric': METRIC,
         'value': value,
@@ -519,7 +565,7 @@ def main():
         'n_gpus': 1,
         'steps': args.steps,
         'warmup': args.warmup,
-        'ms_per_step': t_fused / args.steps * 1e3,
+        'ms_per_step': step_s * 1e3,
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
@@ -530,17 +576,16 @@ def main():
                         'gate, seed %d' % (n, G, args.seed),
             'path': 'fused fp32 (%s partition, %d passes)' % (method, len(fused)),
             'fusion': method,
-            'plain_fp64_gates_per_s': gps(t_plain),
-            'other_paths_gates_per_s': {k: gps(v[0]) for k, v in extra.items()},
-            'speedup_fused_fp32_vs_plain_fp64': value / gps(t_plain),
+            'plain_fp64_gates_per_s': plain_gps,
+            'plain_fp64_timed_applications': args.plain_steps,
+            'speedup_fused_fp32_vs_plain_fp64': value / plain_gps,
+            'plan_host_s': plan_s,
+            'end_to_end_gates_per_s_incl_plan': G / (step_s + plan_s),
             'plain_fp64_dominant_kernel': pdom,
             'plain_fp64_roofline': proof,
             'plain_fp64_streaming_hbm_frac': streaming_fraction(f_plain, n, 'fp64', None),
-            'plain_fp32_streaming_hbm_frac': (streaming_fraction(extra['plain_fp32'][1], n, 'fp32', None)
-                                              if 'plain_fp32' in extra else None),
             'fused_fp32_kernel_ms': {k: v[1] / max(1, v[0]) for k, v in f_fused.items() if v[0]},
             'fused_fp32_launches': {k: v[0] for k, v in f_fused.items() if v[0]},
-            'plan_host_s': plan_s,
             'accuracy_fp32_vs_fp64': acc,
         },
         'roofline': dict(roof, kernel=dom, avg_launch_ms=avg_s * 1e3, algorithmic_bytes=byts,
@@ -548,6 +593,23 @@ def main():
         'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)
+
+    if args.all_paths:
+        # the other forms of the same workload, 1 warm-up + 2 timed steps each,
+        # reported on stderr after the headline line
+        extra = {}
+        log('timing plain fp32, fused fp64, greedy dense-MFMA fp32 ...')
+        extra['plain_fp32'] = time_program(plain, n, 'fp32', 2, 1)
+        extra['fused_fp64'] = time_program(fused, n, 'fp64', 2, 1)
+        dense, _ = fusion.fused_program(noisy, n, 4, 'greedy', None, 'dense')
+        extra['greedy_dense_fp32'] = time_program(dense, n, 'fp32', 2, 1)
+        dense3, _ = fusion.fused_program(noisy, n, 3, 'greedy', None, 'dense')
+        extra['greedy_dense_k3_fp32'] = time_program(dense3, n, 'fp32', 2, 1)
+        if method != 'greedy':
+            gseq, _ = fusion.fused_program(noisy, n, 4, 'greedy', None, 'seq')
+            extra['greedy_seq_fp32'] = time_program(gseq, n, 'fp32', 2, 1)
+        log('other paths (gates/s): %s' % json.dumps({k: G * 2 / v[0] for k, v in extra.items()}))
+        log('plain fp32 streaming HBM fraction: %s' % streaming_fraction(extra['plain_fp32'][1], n, 'fp32', None))
 
 
 if __name__ == '__main__':

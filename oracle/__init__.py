@@ -15,24 +15,28 @@ Contents
              plus the host scheduling (``single_gate_merge``, ``partition``).
 ``dense``    Brute-force 2^n x 2^n complex density matrix with Kraus
              channels.  It pins the Pauli-basis restatement by physics.
-``ptm``      Pauli transfer matrices of the channels.
+``mlp``      numpy forward / backward of the fusion cost model's perceptron.
 
-Pinning status (each function's own docstring names its pin):
+Pinning status (the tests named here are the pins):
 
-* gate/channel updates (U3, CX with and without TSP error, memory noise,
-  reset, X/Y/Z/N collapse, per-gate depolarising): pinned by the dense
-  Kraus oracle (``tests/test_oracle_dense.py``) for n <= 5;
-* ensemble / partial probabilities: pinned by dense diagonals and the explicit
-  Kronecker operator the reference builds, at n <= 8;
-* Pauli -> matrix conversion: pinned by the dense construction and README /
-  notebook fixtures;
-* merge / partition: pinned by the notebook partition tables
-  (``tests/golden/notebook_fixtures.json``) and, semantically, by the dense
-  oracle (merged U3 equals the product of its gates up to a phase);
-* TSP-error CX with <cos> < 1: the moment substitution <cos 2a> = 4<cos a> - 3
-  is pinned only by the formula text (basicaertools.py:329-336): parity
-  unpinned beyond the <cos> = 1 case and a linearity check;
-* Bell measurement: pinned by the notebook value
-  (measurement.ipynb cell 10) and by the mirrored-stride rule read from the
-  code text (dm_simulator.py:741); parity unpinned for n > 3 beyond that.
+* gate/channel updates (U3, CX without TSP error and with it at <cos> = 1,
+  memory noise, reset, X/Y/Z/N collapse, per-gate depolarising): the dense
+  Kraus oracle, n <= 3 (``tests/test_oracle.py``);
+* TSP-error CX with <cos> < 1: the documented U(a) channel averaged exactly
+  over two symmetric angle distributions with the same <cos delta>, affinely
+  extrapolated to the reference's substitution <cos 2 delta> = 4 <cos delta> - 3
+  (``tests/test_oracle_pins.py``, with planted coefficient mistakes);
+* Bell measurement: dense Bell projectors on the mirrored pair (qubits
+  n-1-q2, n-1-q1) of random mixed states at n = 4, 5, both pair orders
+  (``tests/test_oracle_pins.py``), and the notebook value
+  (measurement.ipynb cell 10);
+* ensemble / partial probabilities: dense diagonals and the explicit
+  Kronecker operator the reference builds (``tests/test_oracle.py``);
+* Pauli -> matrix conversion: the dense construction and README / notebook
+  fixtures;
+* merge / partition: the notebook partition tables
+  (``tests/golden/notebook_fixtures.json``) and, semantically, the dense
+  oracle (a merged U3 equals the product of its gates up to a phase);
+* the perceptron (``mlp``): central finite differences
+  (``tests/test_oracle_mlp.py``).
 """

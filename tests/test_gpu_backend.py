@@ -168,6 +168,29 @@ def test_measure_modes_fp64():
             assert np.max(np.abs(got['reduced_bell_densitymatrix02'] - ref['reduced_bell_densitymatrix02'])) <= TOL
 
 
+@pytest.mark.parametrize('n,pair', [(4, '13'), (4, '31'), (5, '04'), (5, '32')])
+def test_bell_measure_asymmetric_states_fp64(n, pair):
+    """Bell measurement after a random noisy circuit (no symmetry between
+    the qubits), both pair orders: probabilities, reduced matrix and the
+    collapsed state against the oracle, whose Bell function is pinned by
+    dense Bell projectors on the mirrored pair."""
+    rng = np.random.default_rng(int(pair) + n)
+    qc = QuantumCircuit(n, n)
+    for _ in range(12):
+        if rng.random() < 0.6:
+            qc.u3(*rng.uniform(0, 2 * np.pi, 3), int(rng.integers(n)))
+        else:
+            c, t = (int(x) for x in rng.choice(n, 2, replace=False))
+            qc.cx(c, t)
+    qc.measure(0, 0, basis='Bell', add_param=pair)
+    ref, got, _ = run_both(qc, {'decay_factor': 0.99, 'decoherence_factor': 0.98, 'thermal_factor': 0.9})
+    compare(ref, got)
+    key = 'bell_probabilities' + pair
+    assert max(abs(got[key][k] - ref[key][k]) for k in ref[key]) <= TOL
+    red = 'reduced_bell_densitymatrix' + pair
+    assert np.max(np.abs(got[red] - ref[red])) <= TOL
+
+
 def test_initial_states_fp64(tmp_path):
     n = 3
     for opts in ({'custom_densitymatrix': 'max_mixed'}, {'custom_densitymatrix': 'uniform_superpos'},
