@@ -1164,27 +1164,37 @@ __global__ __launch_bounds__(kBlock) void k_convert(const TS *__restrict__ s, TD
 // (row, col) with qubit 0 the most significant bit.
 // ---------------------------------------------------------------------------
 
+// First qubit: real coefficients in, complex values out (out of place).
 template <typename TS>
-__global__ __launch_bounds__(kBlock) void k_dense_pass(const TS *__restrict__ in, double2 *__restrict__ out,
-                                                       uint64_t groups, int p, int in_complex) {
+__global__ __launch_bounds__(kBlock) void k_dense_first(const TS *__restrict__ in, double2 *__restrict__ out,
+                                                        uint64_t groups, int p) {
   const uint64_t S = pow4(p);
   const uint64_t step = (uint64_t)gridDim.x * kBlock;
   for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < groups; g += step) {
     const uint64_t hi = g >> (2 * p), lo = g & (S - 1);
     const uint64_t b = hi * 4 * S + lo;
-    double2 x[4];
-    for (int c = 0; c < 4; ++c) {
-      if (in_complex) {
-        x[c] = reinterpret_cast<const double2 *>(in)[b + c * S];
-      } else {
-        x[c] = make_double2((double)in[b + c * S], 0.0);
-      }
-    }
+    const double x0 = (double)in[b], x1 = (double)in[b + S], x2 = (double)in[b + 2 * S], x3 = (double)in[b + 3 * S];
     // mu0 = I + Z; mu1 = X - iY; mu2 = X + iY; mu3 = I - Z
-    out[b] = make_double2(x[0].x + x[3].x, x[0].y + x[3].y);
-    out[b + S] = make_double2(x[1].x + x[2].y, x[1].y - x[2].x);
-    out[b + 2 * S] = make_double2(x[1].x - x[2].y, x[1].y + x[2].x);
-    out[b + 3 * S] = make_double2(x[0].x - x[3].x, x[0].y - x[3].y);
+    out[b] = make_double2(x0 + x3, 0.0);
+    out[b + S] = make_double2(x1, -x2);
+    out[b + 2 * S] = make_double2(x1, x2);
+    out[b + 3 * S] = make_double2(x0 - x3, 0.0);
+  }
+}
+
+// Every other qubit, in place on the complex buffer: each thread owns the
+// four entries of its group.
+__global__ __launch_bounds__(kBlock) void k_dense_pass(double2 *__restrict__ buf, uint64_t groups, int p) {
+  const uint64_t S = pow4(p);
+  const uint64_t step = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < groups; g += step) {
+    const uint64_t hi = g >> (2 * p), lo = g & (S - 1);
+    const uint64_t b = hi * 4 * S + lo;
+    const double2 x0 = buf[b], x1 = buf[b + S], x2 = buf[b + 2 * S], x3 = buf[b + 3 * S];
+    buf[b] = make_double2(x0.x + x3.x, x0.y + x3.y);
+    buf[b + S] = make_double2(x1.x + x2.y, x1.y - x2.x);
+    buf[b + 2 * S] = make_double2(x1.x - x2.y, x1.y + x2.x);
+    buf[b + 3 * S] = make_double2(x0.x - x3.x, x0.y - x3.y);
   }
 }
 
@@ -1199,14 +1209,17 @@ __device__ __forceinline__ uint64_t spread_bits(uint64_t x) {
   return x;
 }
 
+// Output entries [o0, o0 + count) of the row-major 2^n x 2^n matrix: entry
+// (row, col) is the buffer value at the bit-interleaved index
+// sum_q (2 r_q + c_q) 4^p(q).
 __global__ __launch_bounds__(kBlock) void k_dense_scatter(const double2 *__restrict__ b, double2 *__restrict__ out,
-                                                          int n) {
+                                                          int n, uint64_t o0, uint64_t count) {
   const uint64_t dim = 1ull << n;
-  const uint64_t total = dim * dim;
   const uint64_t step = (uint64_t)gridDim.x * kBlock;
-  for (uint64_t o = (uint64_t)blockIdx.x * kBlock + threadIdx.x; o < total; o += step) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += step) {
+    const uint64_t o = o0 + i;
     const uint64_t row = o >> n, col = o & (dim - 1);
-    out[o] = b[(spread_bits(row) << 1) | spread_bits(col)];
+    out[i] = b[(spread_bits(row) << 1) | spread_bits(col)];
   }
 }
 
@@ -1797,41 +1810,49 @@ int do_probabilities(dm_state *h, const int *qubits, int m, char basis, const do
   return DM_OK;
 }
 
+// Largest staging chunk of the dense output (bytes): the n per-qubit passes
+// run in place on one complex buffer of 4^n entries, then the scatter writes
+// the row-major matrix through this chunk to the host.
+constexpr uint64_t kDenseChunkBytes = 1ull << 31;
+
 template <typename T>
 int do_to_dense(dm_state *h, double *out_c128) {
   const uint64_t N = h->N;
   const size_t bytes = N * sizeof(double2);
-  double2 *b0 = nullptr, *b1 = nullptr;
-  HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&b0), bytes, h->stream));
-  hipError_t e1 = hipMallocAsync(reinterpret_cast<void **>(&b1), bytes, h->stream);
+  const uint64_t chunk = std::min<uint64_t>(N, kDenseChunkBytes / sizeof(double2));
+  double2 *buf = nullptr, *stage = nullptr;
+  HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&buf), bytes, h->stream));
+  hipError_t e1 = hipMallocAsync(reinterpret_cast<void **>(&stage), chunk * sizeof(double2), h->stream);
   if (e1 != hipSuccess) {
-    hipFreeAsync(b0, h->stream);
-    return fail(DM_ERR_HIP, "hipMallocAsync(dense scratch) failed: %s", hipGetErrorString(e1));
+    hipFreeAsync(buf, h->stream);
+    hipStreamSynchronize(h->stream);
+    return fail(DM_ERR_HIP, "dense output of %d qubits: device buffer of %.1f GB: %s", h->n, bytes / 1e9,
+                hipGetErrorString(e1));
   }
   const uint64_t groups = N / 4;
   ProfScope ps(h, 6);
-  // qubit order 0..n-1, i.e. position n-1 down to 0 (any order gives the same result)
-  const double2 *src = nullptr;
-  double2 *dst = b0;
+  // qubit order 0..n-1, i.e. position n-1 down to 0 (the per-qubit maps commute)
   for (int q = 0; q < h->n; ++q) {
     const int p = h->n - 1 - q;
     if (q == 0)
-      hipLaunchKernelGGL((k_dense_pass<T>), dim3(grid_for(h, groups)), dim3(kBlock), 0, h->stream,
-                         static_cast<const T *>(h->d_a), dst, groups, p, 0);
+      hipLaunchKernelGGL((k_dense_first<T>), dim3(grid_for(h, groups)), dim3(kBlock), 0, h->stream,
+                         static_cast<const T *>(h->d_a), buf, groups, p);
     else
-      hipLaunchKernelGGL((k_dense_pass<double>), dim3(grid_for(h, groups)), dim3(kBlock), 0, h->stream,
-                         reinterpret_cast<const double *>(src), dst, groups, p, 1);
-    src = dst;
-    dst = (dst == b0) ? b1 : b0;
+      hipLaunchKernelGGL(k_dense_pass, dim3(grid_for(h, groups)), dim3(kBlock), 0, h->stream, buf, groups, p);
   }
-  hipLaunchKernelGGL(k_dense_scatter, dim3(grid_for(h, N)), dim3(kBlock), 0, h->stream, src, dst, h->n);
   int rc = last_launch("k_dense");
-  if (rc == DM_OK) {
-    hipError_t e = hipMemcpyAsync(out_c128, dst, bytes, hipMemcpyDeviceToHost, h->stream);
+  for (uint64_t o0 = 0; rc == DM_OK && o0 < N; o0 += chunk) {
+    const uint64_t cnt = std::min<uint64_t>(chunk, N - o0);
+    hipLaunchKernelGGL(k_dense_scatter, dim3(grid_for(h, cnt)), dim3(kBlock), 0, h->stream, buf, stage, h->n, o0,
+                       cnt);
+    rc = last_launch("k_dense_scatter");
+    if (rc) break;
+    hipError_t e = hipMemcpyAsync(reinterpret_cast<double2 *>(out_c128) + o0, stage, cnt * sizeof(double2),
+                                  hipMemcpyDeviceToHost, h->stream);
     if (e != hipSuccess) rc = fail(DM_ERR_HIP, "dense download failed: %s", hipGetErrorString(e));
   }
-  hipFreeAsync(b0, h->stream);
-  hipFreeAsync(b1, h->stream);
+  hipFreeAsync(buf, h->stream);
+  hipFreeAsync(stage, h->stream);
   HIP_TRY(hipStreamSynchronize(h->stream));
   return rc;
 }
@@ -2084,7 +2105,8 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
 #define TILE_LAUNCH_L(MM, D, G, NTH, L)                                                                          \
   do {                                                                                                           \
     const int per_cu = tile_resident_per_cu<T, MM, D, G, NTH, L>();                                             \
-    const bool long_runs = fp.geo.w >= 3 && fp.geo.tiles <= (1ull << 30);                                     \
+    /* one workgroup per tile only where tiles x NTH work items fit a grid */                                 \
+    const bool long_runs = fp.geo.w >= 3 && fp.geo.tiles * (uint64_t)NTH <= 0xffffffffull;                    \
     const int s_cu = D ? 0 : stream_resident_per_cu<T, MM, G, NTH, L>();                                        \
     if (long_runs && !D && sizeof(T) == 4 && (s_cu * NTH / 64 >= 28 || (per_cu < 2 && s_cu >= 2))) {            \
       hipLaunchKernelGGL((k_tile_stream<T, MM, G, NTH, L>), dim3((unsigned)fp.geo.tiles), dim3(NTH), 0,          \
@@ -2294,6 +2316,8 @@ struct dm_batch {
 // MI355X without an error message, and graph capture at that size stays off
 // until the cause is known.
 constexpr int64_t kBatchGraphLaunchLimit = 16384;
+// HBM-resident circuits per batch (dm_batch_create refuses more).
+constexpr int kBatchMaxHbmCircuits = 12;
 
 namespace {
 
@@ -2354,6 +2378,7 @@ int small_circuit_ops(int n, const dm_op *ops, int nops, const double *params, i
           const int nv = (int)p[at + 6];
           at += 7;
           if (nv < 0 || at + nv > nparams) return fail(DM_ERR_ARG, "circuit %d op %d: bad fused values", ci, i);
+          if (kq < 0 || kq > 4) return fail(DM_ERR_ARG, "circuit %d op %d: bad fused sub-op size %d", ci, i, kq);
           for (int x = 0; x < kq; ++x)
             if (sq[x] < 0 || sq[x] >= n) return fail(DM_ERR_ARG, "circuit %d op %d: bad qubit", ci, i);
           if (kind == TOP_1Q && kq == 1 && nv == 12) push(TOP_1Q, 1, sq, p + at, 12);
@@ -2893,6 +2918,15 @@ int dm_batch_create(int dtype, int ncirc, const int *n_qubits, const int64_t *op
     return fail(DM_ERR_ARG, "lds_threads must be 256, 512 or 1024");
   if (lds_max_qubits > 7) return fail(DM_ERR_ARG, "LDS circuits hold at most 7 qubits");
   if (dtype == DM_FP64 && lds_max_qubits > 7) return fail(DM_ERR_ARG, "bad LDS width");
+  {
+    // circuits that keep their state in HBM, counted before anything is
+    // allocated (each has its own state, program and per-pass launches)
+    int hbm = 0;
+    for (int c = 0; c < ncirc; ++c) hbm += n_qubits[c] > lds_max_qubits ? 1 : 0;
+    if (hbm > kBatchMaxHbmCircuits)
+      return fail(DM_ERR_ARG, "batch holds %d circuits above the LDS width %d; at most %d are supported", hbm,
+                  lds_max_qubits, kBatchMaxHbmCircuits);
+  }
   dm_batch *b = new dm_batch();
   b->dtype = dtype;
   b->ncirc = ncirc;
@@ -2923,8 +2957,8 @@ int dm_batch_create(int dtype, int ncirc, const int *n_qubits, const int64_t *op
 
 int dm_batch_run(dm_batch *b, int use_graph, double *probs_out, double *circuit_us, double *total_ms) {
   if (!b) return fail(DM_ERR_ARG, "null batch");
-  if (use_graph && b->launches > kBatchGraphLaunchLimit)
-    return fail(DM_ERR_ARG, "batch run of %lld kernel launches exceeds the graph capture limit (%lld)",
+  if (b->launches > kBatchGraphLaunchLimit)
+    return fail(DM_ERR_ARG, "batch run of %lld kernel launches exceeds the limit (%lld)",
                 (long long)b->launches, (long long)kBatchGraphLaunchLimit);
   if (use_graph && !b->exec) {
     HIP_TRY(hipStreamBeginCapture(b->master, hipStreamCaptureModeThreadLocal));

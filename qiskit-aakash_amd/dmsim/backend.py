@@ -22,7 +22,6 @@ import itertools
 import logging
 import math
 import os
-import threading
 import time
 import types
 import uuid
@@ -33,6 +32,22 @@ from . import native, ptm, schedule
 from .exceptions import BasicAerError
 
 logger = logging.getLogger(__name__)
+
+
+# device staging buffer of the dense output's scatter (dm_to_dense)
+DENSE_CHUNK_BYTES = 1 << 31
+
+
+def _host_available_bytes():
+    """MemAvailable of this host, or None where it cannot be read."""
+    try:
+        with open('/proc/meminfo') as f:
+            for line in f:
+                if line.startswith('MemAvailable:'):
+                    return int(line.split()[1]) * 1024
+    except OSError:
+        pass
+    return None
 
 
 def _hbm_max_qubits():
@@ -144,7 +159,6 @@ class DmSimulator:
         self._precision = 'fp64'
         self._gate_depolarization = 0.0
         self._fusion = 'none'
-        self._lock = threading.Lock()
 
     # identity -------------------------------------------------------------------
     def name(self):
@@ -160,6 +174,18 @@ class DmSimulator:
 
     def provider(self):
         return self._provider
+
+    def properties(self):
+        """No calibration data (basebackend.py:61-68: a simulator returns None)."""
+        return None
+
+    def status(self):
+        """Backend status with the fields of basebackend.py:78-90."""
+        st = {'backend_name': self.name(), 'backend_version': self.DEFAULT_CONFIGURATION['backend_version'],
+              'operational': True, 'pending_jobs': 0, 'status_msg': ''}
+        ns = types.SimpleNamespace(**st)
+        ns.to_dict = lambda s=st: dict(s)
+        return ns
 
     def __repr__(self):
         return "<DmSimulator('dm_simulator')>"
@@ -253,10 +279,25 @@ class DmSimulator:
 
     # job plumbing -------------------------------------------------------------------------
     def run(self, qobj, backend_options=None):
+        """Set the options on this (shared) backend, as the reference does
+        (dm_simulator.py:889-919), then run the job on a snapshot of them:
+        the reference's process-pool job pickles the backend at submission,
+        so a later run() cannot change the options of a queued job."""
         self._set_options(qobj_config=getattr(qobj, 'config', None), backend_options=backend_options)
-        job = DmSimJob(self, str(uuid.uuid4()), self._run_job, qobj)
+        job = DmSimJob(self, str(uuid.uuid4()), self._snapshot()._run_job, qobj)
         job.submit()
         return job
+
+    def _snapshot(self):
+        """A copy of the backend carrying the options in force now; option
+        containers that later runs modify in place (the shared rotation-error
+        dict, lists, arrays) are copied too."""
+        snap = copy.copy(self)
+        for attr in ('_rotation_error', '_tsp_model_error', '_initial_densitymatrix', '_density_matrix_stored'):
+            val = getattr(self, attr, None)
+            if isinstance(val, (dict, list, np.ndarray)):
+                setattr(snap, attr, copy.deepcopy(val))
+        return snap
 
     def _validate(self, qobj):
         n = qobj.config.n_qubits
@@ -269,8 +310,32 @@ class DmSimulator:
                 logger.warning('No measurements in circuit "%s", classical register will remain all zeros.',
                                exp.header.name)
 
+    def _check_dense_output(self, n):
+        """The densitymatrix output of n qubits is a 2^n x 2^n complex128
+        array (16 * 4^n bytes) on the host, built on the device in one buffer
+        of that size next to the state (dm_to_dense).  Refuse before the
+        simulation runs when either does not fit."""
+        dense = 16 * 4 ** n
+        state = (8 if self._precision == 'fp64' else 4) * 4 ** n
+        try:
+            hbm = native.device_info(0)['hbm_bytes']
+        except Exception:  # no device visible: the run itself will report it
+            hbm = None
+        host = _host_available_bytes()
+        need_dev = dense + state + min(dense, DENSE_CHUNK_BYTES)
+        if (hbm is not None and need_dev > 0.95 * hbm) or (host is not None and dense > host):
+            raise BasicAerError(
+                'The densitymatrix of {} qubits needs {:.1f} GB on the host and {:.1f} GB of device memory '
+                '(available: {} host, {} device); run with compute_densitymatrix=False for the Pauli '
+                'coefficients only.'.format(n, dense / 1e9, need_dev / 1e9,
+                                            'unknown' if host is None else '%.1f GB' % (host / 1e9),
+                                            'unknown' if hbm is None else '%.1f GB' % (hbm / 1e9)))
+
     def _run_job(self, job_id, qobj):
         self._validate(qobj)
+        if self._get_den_mat:
+            for exp in qobj.experiments:
+                self._check_dense_output(exp.config.n_qubits)
         t0 = time.time()
         results = [self.run_experiment(exp) for exp in qobj.experiments]
         t1 = time.time()

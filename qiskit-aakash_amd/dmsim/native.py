@@ -9,6 +9,8 @@ import threading
 
 import numpy as np
 
+from .exceptions import BasicAerError
+
 _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libdmsim.so')
 _lib = None
 _lock = threading.Lock()
@@ -81,8 +83,11 @@ FAMILIES = {'1q': 0, 'cx': 1, '2q': 2, 'memory_noise': 3, 'diag': 4, 'block3': 5
             'block4': 8, 'fused': 9}
 
 
-class NativeError(RuntimeError):
-    """A failure reported by libdmsim (message from dm_last_error)."""
+class NativeError(BasicAerError, RuntimeError):
+    """A failure reported by libdmsim (message from dm_last_error).  It is a
+    BasicAerError, the error type the reference backend raises, so callers
+    catching the reference's errors catch device failures (out of HBM, bad
+    arguments) too."""
 
 
 def library_path():
