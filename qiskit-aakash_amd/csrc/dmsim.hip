@@ -1376,10 +1376,8 @@ struct dm_state {
   // scratch
   double *d_red = nullptr;      // reduction partials + results
   size_t red_cap = 0;           // doubles
-  void *d_mat = nullptr;        // block matrices (transposed, storage dtype)
+  void *d_mat = nullptr;        // program image: tile-operation records, block matrices (storage dtype)
   size_t mat_cap = 0;           // bytes
-  void *d_ops = nullptr;        // tile-program operations
-  size_t ops_cap = 0;           // bytes
   double *d_aux = nullptr;      // gathered vectors / overlap operand
   size_t aux_cap = 0;           // bytes
   // profiling
@@ -2137,8 +2135,12 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
   return last_launch("k_tile_prog");
 }
 
-// A program split into an upload phase (prepare) and a launch phase that only
-// enqueues kernels on the handle's stream (so it can be captured in a graph).
+// A program split into a host phase (stage: validate, parse the fused
+// passes, lay out every tile-operation record and block matrix in one host
+// image), an upload phase (the image into device memory) and a launch phase
+// that only enqueues kernels on the handle's stream (so it can be captured in
+// a graph).  The device image is [tile-operation records | values (T)], the
+// values starting at a 256-byte boundary.
 struct Prepared {
   std::vector<dm_op> ops;
   std::vector<double> params;
@@ -2146,10 +2148,16 @@ struct Prepared {
   std::vector<int> fused_of;
   std::vector<size_t> offs, top_off;
   std::vector<int> pos;
+  std::vector<TileOp> tops;            // host image: records
+  std::vector<unsigned char> values;   // host image: values, storage type
+  size_t vals_at = 0;                  // byte offset of the values in the image
+  const void *d_mat = nullptr;         // device values (after upload)
+  const void *d_ops = nullptr;         // device records (after upload)
+  size_t image_bytes() const { return vals_at + values.size(); }
 };
 
 template <typename T>
-int prepare_program(dm_state *h, const dm_op *ops, int nops, const double *params, int64_t nparams, Prepared *pp) {
+int stage_program(dm_state *h, const dm_op *ops, int nops, const double *params, int64_t nparams, Prepared *pp) {
   pp->ops.assign(ops, ops + nops);
   pp->params.assign(params, params + (nparams > 0 ? nparams : 0));
   pp->fused.clear();
@@ -2172,8 +2180,8 @@ int prepare_program(dm_state *h, const dm_op *ops, int nops, const double *param
   pp->top_off.assign(pp->fused.size(), 0);
   // Tile operations are gathered for every fused pass, also when no pass
   // has parameter values (noiseless CX only): the kernels read them from
-  // d_ops.
-  std::vector<TileOp> tops;
+  // the records.
+  pp->tops.clear();
   std::vector<T> staging(pool);
   size_t off = 0;
   for (int i = 0; i < nops; ++i) {
@@ -2186,29 +2194,45 @@ int prepare_program(dm_state *h, const dm_op *ops, int nops, const double *param
       FusedPass &fp = pp->fused[pp->fused_of[i]];
       pp->offs[i] = off;
       for (double v : fp.vals) staging[off++] = (T)v;
-      pp->top_off[pp->fused_of[i]] = tops.size();
-      tops.insert(tops.end(), fp.ops.begin(), fp.ops.end());
+      pp->top_off[pp->fused_of[i]] = pp->tops.size();
+      pp->tops.insert(pp->tops.end(), fp.ops.begin(), fp.ops.end());
     }
   }
-  if (pool || !tops.empty()) {
-    // a valid (if unread) value pointer for passes without values
-    if (int rc = ensure(&h->d_mat, &h->mat_cap, std::max<size_t>(pool, 1) * sizeof(T))) return rc;
-  }
-  if (pool) {
-    HIP_TRY(hipStreamSynchronize(h->stream));  // earlier kernels may still read d_mat / d_ops
-    HIP_TRY(hipMemcpy(h->d_mat, staging.data(), pool * sizeof(T), hipMemcpyHostToDevice));
-  }
-  if (!tops.empty()) {
-    if (int rc = ensure(&h->d_ops, &h->ops_cap, tops.size() * sizeof(TileOp))) return rc;
-    HIP_TRY(hipStreamSynchronize(h->stream));  // earlier tile passes may still read d_ops
-    HIP_TRY(hipMemcpy(h->d_ops, tops.data(), tops.size() * sizeof(TileOp), hipMemcpyHostToDevice));
-  }
+  pp->vals_at = (pp->tops.size() * sizeof(TileOp) + 255) & ~(size_t)255;
+  pp->values.resize(staging.size() * sizeof(T));
+  if (!staging.empty()) memcpy(pp->values.data(), staging.data(), pp->values.size());
   return DM_OK;
+}
+
+// Copies the host image to dev (the caller owns dev, at least
+// image_bytes() long) after the handle's earlier work, which may still read
+// the previous image there, has finished.
+int upload_image(dm_state *h, Prepared *pp, void *dev) {
+  if (pp->tops.empty() && pp->values.empty()) return DM_OK;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (!pp->tops.empty())
+    HIP_TRY(hipMemcpy(dev, pp->tops.data(), pp->tops.size() * sizeof(TileOp), hipMemcpyHostToDevice));
+  if (!pp->values.empty())
+    HIP_TRY(hipMemcpy(static_cast<char *>(dev) + pp->vals_at, pp->values.data(), pp->values.size(),
+                      hipMemcpyHostToDevice));
+  pp->d_ops = dev;
+  pp->d_mat = static_cast<const char *>(dev) + pp->vals_at;
+  return DM_OK;
+}
+
+// Stage, then upload into the handle's own buffer (grown with hipMalloc).
+template <typename T>
+int prepare_program(dm_state *h, const dm_op *ops, int nops, const double *params, int64_t nparams, Prepared *pp) {
+  if (int rc = stage_program<T>(h, ops, nops, params, nparams, pp)) return rc;
+  if (pp->tops.empty() && pp->values.empty()) return DM_OK;
+  if (h->mat_cap < pp->image_bytes()) HIP_TRY(hipStreamSynchronize(h->stream));  // the old buffer may be in use
+  if (int rc = ensure(&h->d_mat, &h->mat_cap, pp->image_bytes())) return rc;
+  return upload_image(h, pp, h->d_mat);
 }
 
 template <typename T>
 int launch_program(dm_state *h, const Prepared &pp) {
-  const T *dmat = static_cast<const T *>(h->d_mat);
+  const T *dmat = static_cast<const T *>(pp.d_mat);
   const double *params = pp.params.data();
   for (size_t i = 0; i < pp.ops.size(); ++i) {
     const dm_op &op = pp.ops[i];
@@ -2222,7 +2246,7 @@ int launch_program(dm_state *h, const Prepared &pp) {
       case DM_OP_BLOCK: rc = launch_block_dev<T>(h, op.k, &pp.pos[4 * i], dmat + pp.offs[i]); break;
       case DM_OP_FUSED: {
         const int f = pp.fused_of[i];
-        rc = launch_fused<T>(h, pp.fused[f], static_cast<const TileOp *>(h->d_ops) + pp.top_off[f],
+        rc = launch_fused<T>(h, pp.fused[f], static_cast<const TileOp *>(pp.d_ops) + pp.top_off[f],
                              dmat + pp.offs[i]);
         break;
       }
@@ -2662,7 +2686,6 @@ int dm_destroy(dm_state *h) {
   if (h->d_a) hipFree(h->d_a);
   if (h->d_red) hipFree(h->d_red);
   if (h->d_mat) hipFree(h->d_mat);
-  if (h->d_ops) hipFree(h->d_ops);
   if (h->d_aux) hipFree(h->d_aux);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
