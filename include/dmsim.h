@@ -67,6 +67,17 @@ int dm_init_product(dm_state *h, const double *per_qubit4, double scale);
 int dm_upload(dm_state *h, const double *host);
 int dm_download(dm_state *h, double *host);
 int dm_download_chop(dm_state *h, double thr, double *host);
+/* Chunked forms for states too large for one host array (checkpoints of
+ * 16-17 qubit states): coefficients [offset, offset + count) to or from a
+ * host buffer of host_dtype (DM_FP64 or DM_FP32; converted on the device in
+ * chunks of at most 256 MB when it differs from the storage type), and the
+ * FP64 dot product sum_i a_i b_i of that range with host values b.
+ * Replace the whole-array np.save / np.load of _store_density_matrix
+ * (dm_simulator.py:1271-1275), the stored-state load (:337-343) and the
+ * overlap of the compare option (:262-271, 1277-1282). */
+int dm_download_range(dm_state *h, uint64_t offset, uint64_t count, int host_dtype, void *host);
+int dm_upload_range(dm_state *h, uint64_t offset, uint64_t count, int host_dtype, const void *host);
+int dm_dot_range(dm_state *h, uint64_t offset, uint64_t count, int host_dtype, const void *host, double *sum);
 /* Read k single coefficients by flat index. */
 int dm_read_coeffs(dm_state *h, const uint64_t *index, int k, double *out);
 
@@ -192,24 +203,100 @@ int dm_mlp_train(float *params, int n_features, int hidden, const float *X, cons
 
 /* Batches of independent circuits (no reference counterpart; the reference
  * runs experiments one after another, dm_simulator.py:935-936).
- * Circuit c has n_qubits[c] qubits and the program ops[op_start[c] ..
+ * Circuit c has n_qubits[c] (1..12) qubits and the program ops[op_start[c] ..
  * op_start[c+1]) (dm_op records over the shared params array).  Each circuit
  * starts in |0...0>, runs its program and ends with the Z-basis ensemble
  * probabilities of all its qubits with polarisation retention meas_err[c]
  * (2^n values, most significant bit = qubit 0, circuits concatenated).
- * Circuits of at most lds_max_qubits (<= 7) qubits run whole inside one
- * workgroup's LDS (one launch per width, lds_threads threads per workgroup);
- * the others keep their state in HBM and run on 8 parallel streams.
- * dm_batch_run with use_graph != 0 captures the whole batch into one hipGraph
- * on first use and replays it (batches of at most 16384 kernel launches per
- * run; larger ones return DM_ERR_ARG); per-circuit times come from GPU clock
- * stamps. */
+ * Circuits of at most lds_max_qubits (5..7) qubits run whole inside one
+ * workgroup's LDS (one launch per width, lds_threads threads per workgroup).
+ * The others keep their state in one HBM arena: each operation becomes a
+ * 6-digit tile pass (DM_OP_FUSED with tile size 6, DM_OP_1Q, DM_OP_CX[_SCALED],
+ * DM_OP_2Q, DM_OP_MEMORY_NOISE; memory noise as 1-qubit channels, 6 qubits
+ * per pass), each pass is split into at most items_per_pass items of
+ * consecutive tiles, and round r (pass r of every circuit, plus the |0..0>
+ * initialisation first and the probabilities last) is one grid-stride launch
+ * over its items.  A run is at most 7 LDS-tier launches plus one launch per
+ * round (the longest circuit's passes + 2); dm_batch_run with use_graph != 0
+ * captures them into one hipGraph on first use and replays it.  Per-circuit
+ * times (circuit_us) are GPU clock stamps from a circuit's initialisation to
+ * its probabilities; circuits run concurrently.  dm_batch_info reports the
+ * HBM-tier circuit, pass and item counts and the launches of one run. */
 typedef struct dm_batch dm_batch;
 int dm_batch_create(int dtype, int ncirc, const int *n_qubits, const int64_t *op_start, const dm_op *ops,
                     const double *params, int64_t nparams, const double *meas_err, int lds_max_qubits,
-                    int lds_threads, dm_batch **out);
+                    int lds_threads, int items_per_pass, dm_batch **out);
+int dm_batch_info(const dm_batch *b, int64_t *hbm_circuits, int64_t *passes, int64_t *items, int64_t *launches);
 int dm_batch_run(dm_batch *b, int use_graph, double *probs_out, double *circuit_us, double *total_ms);
 int dm_batch_destroy(dm_batch *b);
+
+/* Host front end (no device work; plain C++ over integer arrays).
+ *
+ * Instruction kinds of the merge and the partition: */
+enum {
+  DM_INST_U1 = 1,          /* u1(lambda): params[0] */
+  DM_INST_U3 = 2,          /* u3(theta, phi, lambda); u2 and U arrive as u3 */
+  DM_INST_ID = 3,          /* id / u0: dropped by the merge */
+  DM_INST_SEPARATOR = 4,   /* cx, measure, bfunc, reset, barrier: flush pending runs */
+  DM_INST_ROT = 5,         /* partition: u1 / u3 on q0 */
+  DM_INST_CX = 6,          /* partition: cx on (q0, q1) */
+  DM_INST_MEASURE = 7,     /* partition: measure of q0 */
+  DM_INST_RESET = 8,       /* partition: reset of q0 */
+  DM_INST_BARRIER = 9,     /* partition: segment boundary */
+  DM_INST_OWN_LEVEL = 10,  /* partition: a Bell / Expect / Ensemble measure; at the head of a
+                              segment the whole segment is one level, elsewhere a measure */
+  DM_INST_HOLD_MEASURE = 11, DM_INST_HOLD_RESET = 12  /* internal placeholders */
+};
+
+/* Single-qubit gate merging: single_gate_merge with merge_gates / mergeU /
+ * U3_merge (basicaertools.py:129-307).  With merge != 0 every run of u1/u3
+ * on one qubit between separators folds into one gate that reuses the run's
+ * first record, and every separator flushes all pending runs in qubit order;
+ * with merge == 0 the rotations pass through.  id / u0 are dropped.  Output
+ * record j: out_src[j] (the input record it reuses), out_kind[j] (U1, U3 or
+ * SEPARATOR), out_params[3j..3j+2].  Angles are evaluated with the C
+ * library's sin, cos, atan2, acos (the reference uses numpy's). */
+int dm_merge_rotations(int ninst, const int32_t *kind, const int32_t *qubit, const double *params, int merge,
+                       int32_t *out_src, int32_t *out_kind, double *out_params, int *nout);
+
+/* Level partition: qubit_stack / partition_helper / partition
+ * (basicaertools.py:536-748).  Barriers split segments; each segment's
+ * per-qubit stacks are consumed greedily (a cx when it heads both stacks; all
+ * measures (resets) together once every stack head is one or its
+ * placeholder).  Output: the levels of all segments in order, each padded to
+ * its segment's deepest stack as the reference's table is:
+ * out_ops[out_level_start[l] .. out_level_start[l+1]) are the instruction
+ * indices of level l (nlevels levels); *count is the number of levels the
+ * experiment loop runs.  out_ops holds ninst entries, out_level_start
+ * 4 * ninst + 4 (each pass of the level loop places an instruction or
+ * stalls, and three stalls in a row are an error). */
+int dm_partition(int n, int ninst, const int32_t *kind, const int32_t *q0, const int32_t *q1, int32_t *out_ops,
+                 int32_t *out_level_start, int *nlevels, int *count);
+
+/* Fusion planner (no reference counterpart): gates are noisy 1- or 2-qubit
+ * PTMs, gate_qubits[2i..2i+1] (second -1 for one qubit).  A block is a set
+ * of gates that can run together: its gates keep their order on every qubit
+ * and every gate left out of it that comes earlier acts on other qubits.
+ * method: DM_PLAN_GREEDY (grow each block to kmax qubits), DM_PLAN_SEARCH
+ * (the candidate with the most gates), DM_PLAN_LEARNED (candidates scored by
+ * their predicted net saving under the cost model mlp_params, with the
+ * lookahead widths lookahead[0..nlookahead)); tiles != 0 adds the tile-wide
+ * candidates of the learned planner.  Output: blocks in execution order,
+ * out_members[out_block_start[b] ..] (gate indices) and
+ * out_qubits[out_qubit_start[b] ..] (qubits in order of first use); out
+ * arrays hold ngates, ngates + 1, 2 * ngates and ngates + 1 entries. */
+enum { DM_PLAN_GREEDY = 0, DM_PLAN_SEARCH = 1, DM_PLAN_LEARNED = 2 };
+int dm_plan_fusion(int n, int ngates, const int32_t *gate_qubits, int method, int kmax, int tiles,
+                   const int32_t *lookahead, int nlookahead, const float *mlp_params, int precision,
+                   int32_t *out_members, int32_t *out_block_start, int32_t *out_qubits, int32_t *out_qubit_start,
+                   int *nblocks);
+/* The cost model's forward pass on the host (FP32, fixed summation order;
+ * the planner's predictions) and its predicted seconds for given blocks
+ * (the same arrays as dm_plan_fusion's output, precision DM_FP64/DM_FP32). */
+int dm_mlp_predict_host(const float *params, int n_features, int hidden, const float *X, int n, float *out);
+int dm_cost_block_seconds(int n, int ngates, const int32_t *gate_qubits, int nblocks, const int32_t *members,
+                          const int32_t *block_start, const int32_t *qubits, const int32_t *qubit_start,
+                          const float *mlp_params, int precision, double *out);
 
 /* Stream control and timing (no reference counterpart). */
 int dm_synchronize(dm_state *h);

@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "dmsim.h"
@@ -1046,8 +1047,8 @@ __global__ __launch_bounds__(kBlock) void k_sumsq(const T *__restrict__ a, uint6
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_dot(const T *__restrict__ a, const double *__restrict__ b,
+template <typename T, typename TB>
+__global__ __launch_bounds__(kBlock) void k_dot(const T *__restrict__ a, const TB *__restrict__ b,
                                                 uint64_t N, double *part) {
   double s = 0.0;
   const uint64_t step = (uint64_t)gridDim.x * kBlock;
@@ -1346,6 +1347,145 @@ __global__ __launch_bounds__(1024) void k_circuit_lds(const BatchCirc *__restric
   for (int x = tid; x < M; x += nt) probs[c.out_off + x] = v[x];
   __syncthreads();
   if (tid == 0) stamp_now(&stamps[2 * blockIdx.x + 1]);
+}
+
+// ---------------------------------------------------------------------------
+// K8 (HBM tier): batched circuits whose state lives in HBM.
+//
+// Each circuit is a chain of passes over its own state in one arena:
+//   BP_INIT   write |0...0> (2^-n on every index whose digits are I or Z),
+//   BP_TILE   one 6-digit tile pass (the tile operations of a fused block,
+//             or a single gate / memory-noise group as a one-pass block),
+//   BP_PROBS  gather the {I,Z}^n sub-lattice (2^n values), Walsh-Hadamard in
+//             LDS (FP64), write the Z-ensemble probabilities.
+// A pass is split into items of consecutive tiles.  Round r holds pass r of
+// every circuit that has one; the items of a round are independent (distinct
+// circuits, or distinct tiles of one pass), and one launch per round works
+// through them with a grid-stride loop.  Rounds follow each other on one
+// stream, so a pass sees the previous pass's writes through the kernel
+// boundary: no workgroup ever waits for another.
+// ---------------------------------------------------------------------------
+
+enum { BP_INIT = 0, BP_TILE = 1, BP_PROBS = 2 };
+
+struct BatchPass {
+  TileGeom geo;       // BP_TILE: the pass's 6-digit tile
+  int32_t kind;
+  int32_t circ;       // HBM-tier circuit index (clock stamps)
+  int32_t n;          // register size
+  int32_t nops;       // BP_TILE: tile operations
+  int64_t top_off;    // BP_TILE: first tile operation
+  int64_t state_off;  // first coefficient of the circuit's state in the arena
+  int64_t out_off;    // BP_PROBS: first probability
+  double err;         // BP_PROBS: measurement polarisation retention
+};
+
+struct BatchItem {
+  uint32_t pass;    // index into the pass table
+  uint32_t tile0;   // first tile (BP_INIT: first 4096-coefficient chunk)
+  uint32_t ntiles;
+  uint32_t pad;
+};
+
+constexpr int kQueueThreads = 256;
+constexpr int kQueueTile = 6;          // tile digits of HBM-tier passes
+constexpr int kQueueMaxQubits = 12;    // 2^12 FP64 probabilities fill the LDS buffer
+constexpr int kQueueLdsBytes = 32768;  // one FP64 6-digit tile, or 2^12 FP64 values
+
+template <typename T, bool SWZ>
+__device__ __forceinline__ void queue_tiles(T *s, T *a, const BatchPass &bp, uint32_t t0, uint32_t nt,
+                                            const TileOp *__restrict__ tops, const T *__restrict__ prm) {
+  constexpr int M = kQueueTile;
+  constexpr int E = 1 << (2 * M);
+  constexpr int QPT = E / 4 / kQueueThreads;
+  using Q = Pack<T, 4>;
+  const int tid = threadIdx.x;
+  const int w = bp.geo.w;
+  const int lomask = (1 << (2 * w)) - 1;
+  for (uint32_t tile = t0; tile < t0 + nt; ++tile) {
+    uint64_t base = 0, r = tile;
+    for (int j = 0; j < bp.geo.nr; ++j) {
+      base |= (r & 3ull) << (2 * bp.geo.rpos[j]);
+      r >>= 2;
+    }
+    Q buf[QPT];
+#pragma unroll
+    for (int i = 0; i < QPT; ++i)
+      buf[i] = ldp<Q>(a + base + tile_offset<T, M>(4 * (tid + i * kQueueThreads), w, lomask, bp.geo));
+#pragma unroll
+    for (int i = 0; i < QPT; ++i) lds_put_quad(s, lds_swz<SWZ>(4 * (tid + i * kQueueThreads)), buf[i]);
+    __syncthreads();
+    tile_ops<T, M, false, true, kQueueThreads, SWZ>(s, tops + bp.top_off, bp.nops, prm);
+#pragma unroll
+    for (int i = 0; i < QPT; ++i) {
+      const int c = tid + i * kQueueThreads;
+      stp(a + base + tile_offset<T, M>(4 * c, w, lomask, bp.geo), lds_get_quad(s, lds_swz<SWZ>(4 * c)));
+    }
+    __syncthreads();  // LDS reads of this tile before the next tile's writes
+  }
+}
+
+// The items [i0, i1) of one round, grid-stride over workgroups.
+template <typename T>
+__global__ __launch_bounds__(kQueueThreads) void k_batch_round(const BatchPass *__restrict__ passes,
+                                                               const BatchItem *__restrict__ items, uint32_t i0,
+                                                               uint32_t i1, const TileOp *__restrict__ tops,
+                                                               const T *__restrict__ prm, T *__restrict__ arena,
+                                                               double *__restrict__ probs,
+                                                               uint64_t *__restrict__ stamps) {
+  __shared__ __align__(32) unsigned char raw[kQueueLdsBytes];
+  const int tid = threadIdx.x;
+  for (uint32_t t = i0 + blockIdx.x; t < i1; t += gridDim.x) {
+    const BatchItem it = items[t];
+    const BatchPass &bp = passes[it.pass];
+    T *a = arena + bp.state_off;
+    if (bp.kind == BP_TILE) {
+      T *s = reinterpret_cast<T *>(raw);
+      if (bp.geo.ident) queue_tiles<T, false>(s, a, bp, it.tile0, it.ntiles, tops, prm);
+      else queue_tiles<T, true>(s, a, bp, it.tile0, it.ntiles, tops, prm);
+    } else if (bp.kind == BP_INIT) {
+      if (tid == 0 && it.tile0 == 0) stamp_now(&stamps[2 * bp.circ]);
+      const T amp = (T)ldexp(1.0, -bp.n);
+      const uint64_t c0 = (uint64_t)it.tile0 << (2 * kQueueTile);
+      const uint64_t c1 = c0 + ((uint64_t)it.ntiles << (2 * kQueueTile));
+      for (uint64_t i = c0 + 4 * tid; i < c1; i += 4 * kQueueThreads) {
+        Pack<T, 4> q;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const uint64_t x = i + c;
+          const uint64_t lo = x & 0x5555555555555555ull, hi = (x >> 1) & 0x5555555555555555ull;
+          q.v[c] = (lo ^ hi) ? T(0) : amp;
+        }
+        stp(a + i, q);
+      }
+    } else {
+      // Z ensemble: bit q of x is the Z/I choice of qubit n-1-q (position q);
+      // the butterfly on bit q applies (u, w) -> (u + e w, u - e w), and the
+      // outcome index has qubit 0 as its most significant bit, as v does
+      double *v = reinterpret_cast<double *>(raw);
+      const int n = bp.n, M = 1 << n;
+      for (int x = tid; x < M; x += kQueueThreads) {
+        uint64_t idx = 0;
+        for (int q = 0; q < n; ++q)
+          if ((x >> q) & 1) idx |= 3ull << (2 * q);
+        v[x] = (double)a[idx];
+      }
+      __syncthreads();
+      for (int q = 0; q < n; ++q) {
+        for (int x = tid; x < M / 2; x += kQueueThreads) {
+          const int lo = x & ((1 << q) - 1);
+          const int b = ((x >> q) << (q + 1)) | lo;
+          const double u = v[b], wv = v[b | (1 << q)];
+          v[b] = u + bp.err * wv;
+          v[b | (1 << q)] = u - bp.err * wv;
+        }
+        __syncthreads();
+      }
+      for (int x = tid; x < M; x += kQueueThreads) probs[bp.out_off + x] = v[x];
+      if (tid == 0) stamp_now(&stamps[2 * bp.circ + 1]);
+    }
+    __syncthreads();  // the LDS buffer is free before the next item
+  }
 }
 
 }  // namespace
@@ -1748,13 +1888,77 @@ int do_dot(dm_state *h, const double *d_b, double *out) {
   int rc = reserve_red(h, (size_t)grid + 8);
   if (rc) return rc;
   ProfScope ps(h, 6);
-  hipLaunchKernelGGL((k_dot<T>), dim3(grid), dim3(kBlock), 0, h->stream, static_cast<const T *>(h->d_a), d_b,
-                     h->N, h->d_red);
+  hipLaunchKernelGGL((k_dot<T, double>), dim3(grid), dim3(kBlock), 0, h->stream, static_cast<const T *>(h->d_a),
+                     d_b, h->N, h->d_red);
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, h->stream, h->d_red, grid, h->d_red + grid);
   rc = last_launch("k_dot");
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(out, h->d_red + grid, sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
+  return DM_OK;
+}
+
+// Host <-> device copies and dot products over the coefficients [off, off +
+// count), the host side in type H (double or float).  Equal types copy
+// directly; otherwise each chunk of at most kStageBytes passes through the
+// handle's staging buffer and is converted on the device.  Nothing here
+// allocates more than one chunk, so a 17-qubit state streams through a few
+// hundred megabytes of device memory.
+constexpr uint64_t kStageBytes = 256ull << 20;
+
+template <typename T, typename H>
+int copy_range(dm_state *h, uint64_t off, uint64_t count, H *host, bool to_host) {
+  T *a = static_cast<T *>(h->d_a) + off;
+  if (std::is_same<T, H>::value) {
+    HIP_TRY(hipMemcpyAsync(to_host ? (void *)host : (void *)a, to_host ? (const void *)a : (const void *)host,
+                           count * sizeof(T), to_host ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return last_launch("copy_range");
+  }
+  const uint64_t chunk = kStageBytes / sizeof(H);
+  HIP_TRY(hipStreamSynchronize(h->stream));  // the staging buffer may still be read by earlier work
+  if (int rc = ensure(reinterpret_cast<void **>(&h->d_aux), &h->aux_cap, std::min(count, chunk) * sizeof(H)))
+    return rc;
+  H *stage = reinterpret_cast<H *>(h->d_aux);
+  for (uint64_t i = 0; i < count; i += chunk) {
+    const uint64_t c = std::min(chunk, count - i);
+    if (to_host) {
+      hipLaunchKernelGGL((k_convert<T, H>), dim3(grid_for(h, c)), dim3(kBlock), 0, h->stream, a + i, stage, c);
+      HIP_TRY(hipMemcpyAsync(host + i, stage, c * sizeof(H), hipMemcpyDeviceToHost, h->stream));
+    } else {
+      HIP_TRY(hipMemcpyAsync(stage, host + i, c * sizeof(H), hipMemcpyHostToDevice, h->stream));
+      hipLaunchKernelGGL((k_convert<H, T>), dim3(grid_for(h, c)), dim3(kBlock), 0, h->stream, stage, a + i, c);
+    }
+    if (int rc = last_launch("k_convert")) return rc;
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return DM_OK;
+}
+
+template <typename T, typename H>
+int dot_range(dm_state *h, uint64_t off, uint64_t count, const H *host, double *out) {
+  const T *a = static_cast<const T *>(h->d_a) + off;
+  const uint64_t chunk = kStageBytes / sizeof(H);
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (int rc = ensure(reinterpret_cast<void **>(&h->d_aux), &h->aux_cap, std::min(count, chunk) * sizeof(H)))
+    return rc;
+  H *stage = reinterpret_cast<H *>(h->d_aux);
+  const int grid = 2048;
+  if (int rc = reserve_red(h, (size_t)grid + 8)) return rc;
+  double total = 0.0;
+  for (uint64_t i = 0; i < count; i += chunk) {
+    const uint64_t c = std::min(chunk, count - i);
+    HIP_TRY(hipMemcpyAsync(stage, host + i, c * sizeof(H), hipMemcpyHostToDevice, h->stream));
+    const int g = std::min(grid_for(h, c), grid);
+    hipLaunchKernelGGL((k_dot<T, H>), dim3(g), dim3(kBlock), 0, h->stream, a + i, stage, c, h->d_red);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, h->stream, h->d_red, g, h->d_red + grid);
+    if (int rc = last_launch("k_dot")) return rc;
+    double part = 0.0;
+    HIP_TRY(hipMemcpyAsync(&part, h->d_red + grid, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    total += part;
+  }
+  *out = total;
   return DM_OK;
 }
 
@@ -2274,33 +2478,6 @@ int run_program(dm_state *h, const dm_op *ops, int nops, const double *params, i
   return launch_program<T>(h, pp);
 }
 
-// Capturable Z-ensemble probabilities of one HBM-resident state into a device
-// buffer (gather + per-qubit contraction).
-template <typename T>
-int enqueue_probs(dm_state *h, double err, double *dst, double *scratch) {
-  GatherSpec gs;
-  memset(&gs, 0, sizeof(gs));
-  gs.m = h->n;
-  gs.base_log = 1;
-  gs.comp = 3;
-  for (int t = 0; t < h->n; ++t) gs.stride[t] = pow4(h->n - 1 - t);
-  const uint64_t count = 1ull << h->n;
-  hipLaunchKernelGGL((k_gather<T>), dim3(grid_for(h, count)), dim3(kBlock), 0, h->stream,
-                     static_cast<const T *>(h->d_a), count, gs, scratch);
-  double *src = scratch, *buf = scratch + count;
-  for (int t = 0; t < h->n; ++t) {
-    const uint64_t pre = 1ull << t, post = 1ull << (h->n - 1 - t);
-    double *out = (t == h->n - 1) ? dst : buf;
-    hipLaunchKernelGGL(k_contract, dim3(grid_for(h, pre * 2 * post)), dim3(kBlock), 0, h->stream, src, out, pre, 2,
-                       post, 1.0, err, 0.0, 0.0, 1.0, -err, 0.0, 0.0);
-    std::swap(src, buf);
-    if (t == h->n - 1) break;
-    src = out;
-    buf = (out == scratch + count) ? scratch : scratch + count;
-  }
-  return last_launch("enqueue_probs");
-}
-
 
 }  // namespace
 
@@ -2311,37 +2488,30 @@ struct dm_batch {
   std::vector<int> nq;
   std::vector<int64_t> prob_off;
   int64_t nprobs = 0;
-  // circuits held entirely in LDS, grouped by width
+  std::vector<double> err;
+  // LDS tier: circuits held entirely in one workgroup's LDS, grouped by width
   std::vector<int> small;
   std::vector<int> small_width_start;  // per width 1..7: first index into small-ordered arrays
   void *d_circs = nullptr, *d_tops = nullptr, *d_vals = nullptr;
   int lds_threads = 256;
-  // circuits whose state lives in HBM
+  // HBM tier: the states in one arena, the pass table, the items in round
+  // order (round r: pass r of every circuit), and the tile operations and
+  // values of every pass
   std::vector<int> large;
-  std::vector<dm_state *> states;
-  std::vector<Prepared> progs;
-  std::vector<double> err;
-  std::vector<double *> scratch;
-  hipStream_t master = nullptr;
-  std::vector<hipStream_t> lanes;
-  hipEvent_t fork = nullptr, ev0 = nullptr, ev1 = nullptr;
-  std::vector<hipEvent_t> joins;
+  void *d_arena = nullptr, *d_passes = nullptr, *d_items = nullptr, *d_qtops = nullptr, *d_qvals = nullptr;
+  std::vector<uint32_t> round_start;  // items of round r: [round_start[r], round_start[r + 1])
+  uint32_t nitems = 0;
+  int64_t npasses = 0;
+  int queue_grid = 0;  // resident workgroups of k_batch_round on the device
+  hipStream_t master = nullptr, side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, ev0 = nullptr, ev1 = nullptr;
   double *d_probs = nullptr;
-  uint64_t *d_stamps = nullptr;
+  uint64_t *d_stamps = nullptr;  // start / end clock per circuit: LDS tier first, then the HBM tier
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double clock_khz = 1.0e5;
   int64_t launches = 0;  // kernel launches one run enqueues
 };
-
-// Largest batch (in kernel launches per run) that dm_batch_run captures into
-// a hipGraph.  A 512-circuit batch of 6-12 qubit circuits with per-gate
-// memory noise enqueues ~4e5 launches; such a run ended the process on
-// MI355X without an error message, and graph capture at that size stays off
-// until the cause is known.
-constexpr int64_t kBatchGraphLaunchLimit = 16384;
-// HBM-resident circuits per batch (dm_batch_create refuses more).
-constexpr int kBatchMaxHbmCircuits = 12;
 
 namespace {
 
@@ -2424,60 +2594,101 @@ size_t lds_bytes(int n, int dtype) {
   return ((size_t)1 << (2 * n)) * (dtype == DM_FP64 ? 8 : 4) + ((size_t)1 << n) * 8;
 }
 
-// Kernel launches the HBM-resident part of one circuit enqueues per run.
-int64_t circuit_launches(const dm_state *h, const Prepared &pp) {
-  int64_t k = 2 + 1 + 1 + h->n;  // clock stamps, |0..0> init, gather + n contractions
-  for (const dm_op &op : pp.ops) {
-    if (op.kind != DM_OP_MEMORY_NOISE) {
-      k += 1;
-      continue;
+// The tile pass of one plain operation (a 1q / CX / 2q PTM): a one-operation
+// fused record over its qubits, parsed like any fused pass.
+int plain_op_pass(const dm_state *h, int kind, int kq, const int *qs, const double *v, int nv, int opi,
+                  FusedPass *fp) {
+  std::vector<double> rec = {(double)kQueueTile, (double)kq};
+  for (int x = 0; x < kq; ++x) rec.push_back(qs[x]);
+  rec.push_back(1.0);
+  rec.push_back(kind);
+  rec.push_back(kq);
+  for (int x = 0; x < 4; ++x) rec.push_back(x < kq ? qs[x] : 0);
+  rec.push_back(nv);
+  rec.insert(rec.end(), v, v + nv);
+  return parse_fused(h, rec.data(), (int64_t)rec.size(), opi, fp);
+}
+
+// The 6-digit tile passes of one program operation of an HBM-tier circuit.
+// Memory noise becomes one 1-qubit channel per qubit, positions 0-5 in the
+// first pass and 5 more per further pass.
+int queue_op_passes(const dm_state *h, const dm_op &op, const double *params, int64_t nparams, int ci, int i,
+                    std::vector<FusedPass> *out) {
+  if (int rc = validate_op(h, op, i, nparams)) return rc;
+  const double *pv = params + op.param;
+  FusedPass fp;
+  int rc = DM_OK;
+  switch (op.kind) {
+    case DM_OP_1Q: rc = plain_op_pass(h, TOP_1Q, 1, op.qubits, pv, 12, i, &fp); break;
+    case DM_OP_CX: rc = plain_op_pass(h, TOP_CX, 2, op.qubits, pv, 0, i, &fp); break;
+    case DM_OP_CX_SCALED: rc = plain_op_pass(h, TOP_CX, 2, op.qubits, pv, 16, i, &fp); break;
+    case DM_OP_2Q: rc = plain_op_pass(h, TOP_2Q, 2, op.qubits, pv, 256, i, &fp); break;
+    case DM_OP_FUSED: {
+      if (op.param < 0 || op.param >= nparams) return fail(DM_ERR_ARG, "circuit %d op %d: bad parameter offset", ci, i);
+      rc = parse_fused(h, params + op.param, nparams - op.param, i, &fp);
+      if (rc) return rc;
+      if (fp.m != kQueueTile)
+        return fail(DM_ERR_ARG, "circuit %d op %d: batched circuits take %d-digit fused tiles, not %d", ci, i,
+                    kQueueTile, fp.m);
+      for (const TileOp &t : fp.ops)
+        if (t.kind == TOP_DENSE) return fail(DM_ERR_ARG, "circuit %d op %d: dense tile operations are not batched", ci, i);
+      break;
     }
-    const double *pv = pp.params.data() + op.param;
-    if (pv[0] == 1.0 && pv[2] == 1.0) continue;
-    k += ((1 - pv[2]) * (2 * pv[1] - 1) == 0.0) ? 1 : 1 + (h->n - 1 + 2) / 3;
+    case DM_OP_MEMORY_NOISE: {
+      const double f = pv[0], p = pv[1], g = pv[2];
+      if (f == 1.0 && g == 1.0) return DM_OK;
+      const double od = sqrt(g) * f, dd = (1 - g) * (2 * p - 1);
+      const double R[12] = {0, od, 0, 0, 0, 0, od, 0, dd, 0, 0, g};
+      for (int lo = 0; lo < h->n; lo = (lo == 0 ? kQueueTile : lo + kQueueTile - 1)) {
+        const int hi = std::min(h->n, lo == 0 ? kQueueTile : lo + kQueueTile - 1);
+        std::vector<double> rec = {(double)kQueueTile, (double)(hi - lo)};
+        for (int pos = lo; pos < hi; ++pos) rec.push_back(h->n - 1 - pos);
+        rec.push_back(hi - lo);
+        for (int pos = lo; pos < hi; ++pos) {
+          const double head[7] = {(double)TOP_1Q, 1.0, (double)(h->n - 1 - pos), 0, 0, 0, 12.0};
+          rec.insert(rec.end(), head, head + 7);
+          rec.insert(rec.end(), R, R + 12);
+        }
+        FusedPass mp;
+        if (int r2 = parse_fused(h, rec.data(), (int64_t)rec.size(), i, &mp)) return r2;
+        out->push_back(std::move(mp));
+      }
+      return DM_OK;
+    }
+    default:
+      return fail(DM_ERR_ARG, "circuit %d op %d: kind %d is not batched", ci, i, op.kind);
   }
-  return k;
+  if (rc) return rc;
+  out->push_back(std::move(fp));
+  return DM_OK;
 }
 
 template <typename T>
 int enqueue_batch(dm_batch *b) {
   HIP_TRY(hipEventRecord(b->fork, b->master));
-  for (auto ln : b->lanes) HIP_TRY(hipStreamWaitEvent(ln, b->fork, 0));
-  // LDS circuits: one launch per width, spread over the lanes
+  HIP_TRY(hipStreamWaitEvent(b->side, b->fork, 0));
+  // LDS tier on the side stream: one launch per width
   for (int w = 1; w <= 7; ++w) {
     const int lo = b->small_width_start[w], hi = b->small_width_start[w + 1];
     if (hi <= lo) continue;
-    hipStream_t st = b->lanes[w % b->lanes.size()];
-    const size_t bytes = lds_bytes(w, b->dtype);
-    hipLaunchKernelGGL((k_circuit_lds<T>), dim3(hi - lo), dim3(b->lds_threads), bytes, st,
+    hipLaunchKernelGGL((k_circuit_lds<T>), dim3(hi - lo), dim3(b->lds_threads), lds_bytes(w, b->dtype), b->side,
                        static_cast<const BatchCirc *>(b->d_circs) + lo, static_cast<const TileOp *>(b->d_tops),
                        static_cast<const T *>(b->d_vals), b->d_probs, b->d_stamps + 2 * (size_t)lo);
     if (int rc = last_launch("k_circuit_lds")) return rc;
   }
-  // HBM circuits: init, program, probabilities, bracketed by clock stamps
-  const size_t ns = b->small.size();
-  for (size_t j = 0; j < b->large.size(); ++j) {
-    dm_state *h = b->states[j];
-    const int c = b->large[j];
-    uint64_t *st = b->d_stamps + 2 * (ns + j);
-    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, h->stream, st);
-    double tab[4 * kMaxQubits];
-    for (int p = 0; p < h->n; ++p) {
-      tab[4 * p] = 1.0;
-      tab[4 * p + 1] = 0.0;
-      tab[4 * p + 2] = 0.0;
-      tab[4 * p + 3] = 1.0;
-    }
-    if (int rc = launch_diag_tab<T>(h, tab, true, ldexp(1.0, -h->n))) return rc;
-    if (int rc = launch_program<T>(h, b->progs[j])) return rc;
-    if (int rc = enqueue_probs<T>(h, b->err[c], b->d_probs + b->prob_off[c], b->scratch[j])) return rc;
-    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, h->stream, st + 1);
+  // HBM tier on the master stream: one launch per round, in order
+  for (size_t r = 0; r + 1 < b->round_start.size(); ++r) {
+    const uint32_t i0 = b->round_start[r], i1 = b->round_start[r + 1];
+    const int grid = (int)std::min<uint32_t>(i1 - i0, (uint32_t)b->queue_grid);
+    hipLaunchKernelGGL((k_batch_round<T>), dim3(grid), dim3(kQueueThreads), 0, b->master,
+                       static_cast<const BatchPass *>(b->d_passes), static_cast<const BatchItem *>(b->d_items), i0,
+                       i1, static_cast<const TileOp *>(b->d_qtops), static_cast<const T *>(b->d_qvals),
+                       static_cast<T *>(b->d_arena), b->d_probs, b->d_stamps + 2 * b->small.size());
+    if (int rc = last_launch("k_batch_round")) return rc;
   }
-  for (size_t l = 0; l < b->lanes.size(); ++l) {
-    HIP_TRY(hipEventRecord(b->joins[l], b->lanes[l]));
-    HIP_TRY(hipStreamWaitEvent(b->master, b->joins[l], 0));
-  }
-  return last_launch("enqueue_batch");
+  HIP_TRY(hipEventRecord(b->join, b->side));
+  HIP_TRY(hipStreamWaitEvent(b->master, b->join, 0));
+  return DM_OK;
 }
 
 }  // namespace
@@ -2495,33 +2706,11 @@ int set_error(int code, const char *msg) {
 
 extern "C" int dm_batch_destroy(dm_batch *b);
 
-static int batch_state(int n, int dtype, hipStream_t stream, int cu, dm_state **out) {
-  dm_state *h = new dm_state();
-  h->n = n;
-  h->dtype = dtype;
-  h->N = pow4(n);
-  h->cu_count = cu;
-  h->stream = stream;
-  h->owns_stream = false;
-  hipGetDevice(&h->device);
-  hipError_t e = hipMalloc(&h->d_a, h->N * (dtype == DM_FP64 ? 8 : 4));
-  if (e == hipSuccess) e = hipEventCreate(&h->ev0);
-  if (e == hipSuccess) e = hipEventCreate(&h->ev1);
-  if (e != hipSuccess) {
-    const int rc = fail(DM_ERR_HIP, "batch state (%d qubits): %s", n, hipGetErrorString(e));
-    dm_destroy(h);
-    return rc;
-  }
-  *out = h;
-  return DM_OK;
-}
-
 template <typename T>
 static int batch_create_t(dm_batch *b, const int64_t *op_start, const dm_op *ops, const double *params,
-                          int64_t nparams, int lds_max_qubits) {
-  int cu = 256;
+                          int64_t nparams, int lds_max_qubits, int items_per_pass) {
+  int cu = 256, dev = 0;
   {
-    int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
       cu = prop.multiProcessorCount;
@@ -2529,19 +2718,6 @@ static int batch_create_t(dm_batch *b, const int64_t *op_start, const dm_op *ops
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
       b->clock_khz = khz;
   }
-  HIP_TRY(hipStreamCreateWithFlags(&b->master, hipStreamNonBlocking));
-  const int nlanes = 8;
-  for (int l = 0; l < nlanes; ++l) {
-    hipStream_t st;
-    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    b->lanes.push_back(st);
-    hipEvent_t e;
-    HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    b->joins.push_back(e);
-  }
-  HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
-  HIP_TRY(hipEventCreate(&b->ev0));
-  HIP_TRY(hipEventCreate(&b->ev1));
   // LDS circuits grouped by width (probabilities stay in circuit order)
   std::vector<BatchCirc> circs;
   std::vector<TileOp> tops;
@@ -2567,34 +2743,129 @@ static int batch_create_t(dm_batch *b, const int64_t *op_start, const dm_op *ops
   }
   b->small_width_start[8] = (int)b->small.size();
   for (int w = 1; w <= 7; ++w) b->launches += b->small_width_start[w + 1] > b->small_width_start[w] ? 1 : 0;
+
+  // HBM tier: every circuit as INIT, its tile passes, PROBS; states in one arena
+  std::vector<BatchPass> passes;
+  std::vector<TileOp> qtops;
+  std::vector<double> qvals;
+  std::vector<std::vector<int>> chain;  // pass-table indices of each HBM circuit, in order
+  std::vector<double> work;             // per HBM circuit: bytes moved (ticket order)
+  uint64_t arena = 0;
+  for (int c = 0; c < b->ncirc; ++c) {
+    if (b->nq[c] <= lds_max_qubits) continue;
+    const int j = (int)b->large.size();
+    b->large.push_back(c);
+    dm_state probe;
+    probe.n = b->nq[c];
+    probe.dtype = b->dtype;
+    probe.N = pow4(probe.n);
+    BatchPass bp;
+    memset(&bp, 0, sizeof(bp));
+    bp.circ = j;
+    bp.n = probe.n;
+    bp.state_off = (int64_t)arena;
+    arena += (probe.N + 63) & ~63ull;  // 256-byte aligned states
+    std::vector<int> ids;
+    bp.kind = BP_INIT;
+    ids.push_back((int)passes.size());
+    passes.push_back(bp);
+    for (int64_t i = op_start[c]; i < op_start[c + 1]; ++i) {
+      std::vector<FusedPass> fps;
+      if (int rc = queue_op_passes(&probe, ops[i], params, nparams, c, (int)(i - op_start[c]), &fps)) return rc;
+      for (FusedPass &fp : fps) {
+        BatchPass tp = bp;
+        tp.kind = BP_TILE;
+        tp.geo = fp.geo;
+        tp.nops = (int32_t)fp.ops.size();
+        tp.top_off = (int64_t)qtops.size();
+        const size_t vbase = qvals.size();
+        for (TileOp t : fp.ops) {
+          t.param += (int32_t)vbase;
+          qtops.push_back(t);
+        }
+        qvals.insert(qvals.end(), fp.vals.begin(), fp.vals.end());
+        ids.push_back((int)passes.size());
+        passes.push_back(tp);
+      }
+    }
+    bp.kind = BP_PROBS;
+    bp.out_off = b->prob_off[c];
+    bp.err = b->err[c];
+    ids.push_back((int)passes.size());
+    passes.push_back(bp);
+    work.push_back((double)ids.size() * (double)probe.N);
+    chain.push_back(std::move(ids));
+  }
+  if (qvals.size() > (size_t)INT32_MAX) return fail(DM_ERR_ARG, "batch: too many tile-operation values");
+  b->npasses = (int64_t)passes.size();
+
+  // Items in round order: round r holds pass r of every circuit that has
+  // one, circuits with the most work first.  A pass of t tiles splits into
+  // items of ceil(t / items_per_pass) tiles.
+  std::vector<BatchItem> items;
+  {
+    std::vector<int> order(chain.size());
+    for (size_t j = 0; j < order.size(); ++j) order[j] = (int)j;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
+    size_t rounds = 0;
+    for (auto &ids : chain) rounds = std::max(rounds, ids.size());
+    for (size_t r = 0; r < rounds; ++r) {
+      b->round_start.push_back((uint32_t)items.size());
+      for (int j : order) {
+        if (r >= chain[j].size()) continue;
+        const int pi = chain[j][r];
+        const BatchPass &bp = passes[pi];
+        const uint64_t units = bp.kind == BP_PROBS ? 1 : pow4(bp.n - kQueueTile);
+        const uint64_t per = std::max<uint64_t>(1, (units + items_per_pass - 1) / items_per_pass);
+        for (uint64_t t0 = 0; t0 < units; t0 += per) {
+          BatchItem it;
+          it.pass = (uint32_t)pi;
+          it.tile0 = (uint32_t)t0;
+          it.ntiles = (uint32_t)std::min<uint64_t>(per, units - t0);
+          it.pad = 0;
+          items.push_back(it);
+        }
+        if (items.size() > (size_t)UINT32_MAX / 2) return fail(DM_ERR_ARG, "batch: too many work items");
+      }
+    }
+    if (rounds) b->round_start.push_back((uint32_t)items.size());
+  }
+  b->nitems = (uint32_t)items.size();
+
+  HIP_TRY(hipStreamCreateWithFlags(&b->master, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
+  HIP_TRY(hipEventCreate(&b->ev0));
+  HIP_TRY(hipEventCreate(&b->ev1));
+  auto upload = [](void **dst, const void *src, size_t bytes) -> int {
+    HIP_TRY(hipMalloc(dst, std::max<size_t>(bytes, 16)));
+    if (bytes) HIP_TRY(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return DM_OK;
+  };
   if (!circs.empty()) {
     std::vector<T> tv(vals.begin(), vals.end());
-    HIP_TRY(hipMalloc(&b->d_circs, circs.size() * sizeof(BatchCirc)));
-    HIP_TRY(hipMemcpy(b->d_circs, circs.data(), circs.size() * sizeof(BatchCirc), hipMemcpyHostToDevice));
-    HIP_TRY(hipMalloc(&b->d_tops, std::max<size_t>(1, tops.size()) * sizeof(TileOp)));
-    if (!tops.empty()) HIP_TRY(hipMemcpy(b->d_tops, tops.data(), tops.size() * sizeof(TileOp), hipMemcpyHostToDevice));
-    HIP_TRY(hipMalloc(&b->d_vals, std::max<size_t>(1, tv.size()) * sizeof(T)));
-    if (!tv.empty()) HIP_TRY(hipMemcpy(b->d_vals, tv.data(), tv.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (int rc = upload(&b->d_circs, circs.data(), circs.size() * sizeof(BatchCirc))) return rc;
+    if (int rc = upload(&b->d_tops, tops.data(), tops.size() * sizeof(TileOp))) return rc;
+    if (int rc = upload(&b->d_vals, tv.data(), tv.size() * sizeof(T))) return rc;
     size_t need = 0;
     for (auto &bc : circs) need = std::max(need, lds_bytes(bc.n, b->dtype));
     HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_circuit_lds<T>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
   }
-  // HBM circuits: own state, prepared program, probability scratch
-  for (int c = 0; c < b->ncirc; ++c) {
-    if (b->nq[c] <= lds_max_qubits) continue;
-    dm_state *h = nullptr;
-    if (int rc = batch_state(b->nq[c], b->dtype, b->lanes[b->large.size() % b->lanes.size()], cu, &h)) return rc;
-    b->states.push_back(h);
-    b->large.push_back(c);
-    Prepared pp;
-    if (int rc = prepare_program<T>(h, ops + op_start[c], (int)(op_start[c + 1] - op_start[c]), params, nparams, &pp))
-      return rc;
-    b->launches += circuit_launches(h, pp);
-    b->progs.push_back(std::move(pp));
-    double *sc = nullptr;
-    HIP_TRY(hipMalloc(&sc, 2 * ((size_t)1 << h->n) * sizeof(double)));
-    b->scratch.push_back(sc);
+  if (b->nitems) {
+    std::vector<T> tv(qvals.begin(), qvals.end());
+    HIP_TRY(hipMalloc(&b->d_arena, arena * sizeof(T)));
+    if (int rc = upload(&b->d_passes, passes.data(), passes.size() * sizeof(BatchPass))) return rc;
+    if (int rc = upload(&b->d_items, items.data(), items.size() * sizeof(BatchItem))) return rc;
+    if (int rc = upload(&b->d_qtops, qtops.data(), qtops.size() * sizeof(TileOp))) return rc;
+    if (int rc = upload(&b->d_qvals, tv.data(), tv.size() * sizeof(T))) return rc;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_round<T>, kQueueThreads, 0) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    b->queue_grid = cu * per_cu;
+    b->launches += (int64_t)b->round_start.size() - 1;
   }
   HIP_TRY(hipMalloc(&b->d_probs, std::max<int64_t>(1, b->nprobs) * sizeof(double)));
   HIP_TRY(hipMalloc(&b->d_stamps, 2 * (size_t)b->ncirc * sizeof(uint64_t)));
@@ -2898,6 +3169,49 @@ int dm_overlap(dm_state *h, const double *other_host, double *out) {
   return rc;
 }
 
+static int check_range(const dm_state *h, uint64_t offset, uint64_t count, int host_dtype, const void *host) {
+  if (int rc = check_handle(h)) return rc;
+  if (host_dtype != DM_FP64 && host_dtype != DM_FP32) return fail(DM_ERR_ARG, "unknown host dtype %d", host_dtype);
+  if (offset > h->N || count > h->N - offset)
+    return fail(DM_ERR_ARG, "range [%llu, +%llu) outside the %llu coefficients", (unsigned long long)offset,
+                (unsigned long long)count, (unsigned long long)h->N);
+  if (count && !host) return fail(DM_ERR_ARG, "null host buffer");
+  return DM_OK;
+}
+
+#define STATE_DISPATCH(fn, H, ...) ((h)->dtype == DM_FP64 ? fn<double, H>(__VA_ARGS__) : fn<float, H>(__VA_ARGS__))
+
+int dm_download_range(dm_state *h, uint64_t offset, uint64_t count, int host_dtype, void *host) {
+  if (int rc = check_range(h, offset, count, host_dtype, host)) return rc;
+  if (!count) return DM_OK;
+  ProfScope ps(h, 7);
+  if (host_dtype == DM_FP64)
+    return STATE_DISPATCH(copy_range, double, h, offset, count, static_cast<double *>(host), true);
+  return STATE_DISPATCH(copy_range, float, h, offset, count, static_cast<float *>(host), true);
+}
+
+int dm_upload_range(dm_state *h, uint64_t offset, uint64_t count, int host_dtype, const void *host) {
+  if (int rc = check_range(h, offset, count, host_dtype, host)) return rc;
+  if (!count) return DM_OK;
+  ProfScope ps(h, 7);
+  if (host_dtype == DM_FP64)
+    return STATE_DISPATCH(copy_range, double, h, offset, count,
+                          const_cast<double *>(static_cast<const double *>(host)), false);
+  return STATE_DISPATCH(copy_range, float, h, offset, count, const_cast<float *>(static_cast<const float *>(host)),
+                        false);
+}
+
+int dm_dot_range(dm_state *h, uint64_t offset, uint64_t count, int host_dtype, const void *host, double *sum) {
+  if (int rc = check_range(h, offset, count, host_dtype, host)) return rc;
+  if (!sum) return fail(DM_ERR_ARG, "null output");
+  *sum = 0.0;
+  if (!count) return DM_OK;
+  if (host_dtype == DM_FP64)
+    return STATE_DISPATCH(dot_range, double, h, offset, count, static_cast<const double *>(host), sum);
+  return STATE_DISPATCH(dot_range, float, h, offset, count, static_cast<const float *>(host), sum);
+}
+#undef STATE_DISPATCH
+
 int dm_to_dense(dm_state *h, double *out_c128) {
   if (int rc = check_handle(h)) return rc;
   if (!out_c128) return fail(DM_ERR_ARG, "null output");
@@ -2914,62 +3228,52 @@ int dm_run_program(dm_state *h, const dm_op *ops, int nops, const double *params
 int dm_batch_destroy(dm_batch *b) {
   if (!b) return DM_OK;
   if (b->master) hipStreamSynchronize(b->master);
-  for (auto ln : b->lanes) hipStreamSynchronize(ln);
+  if (b->side) hipStreamSynchronize(b->side);
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
-  for (auto h : b->states) dm_destroy(h);
-  for (auto p : b->scratch) hipFree(p);
-  for (void *p : {b->d_circs, b->d_tops, b->d_vals, (void *)b->d_probs, (void *)b->d_stamps})
+  for (void *p : {b->d_circs, b->d_tops, b->d_vals, b->d_arena, b->d_passes, b->d_items, b->d_qtops, b->d_qvals,
+                  (void *)b->d_probs, (void *)b->d_stamps})
     if (p) hipFree(p);
-  for (auto e : b->joins) hipEventDestroy(e);
-  for (hipEvent_t e : {b->fork, b->ev0, b->ev1})
+  for (hipEvent_t e : {b->fork, b->join, b->ev0, b->ev1})
     if (e) hipEventDestroy(e);
-  for (auto ln : b->lanes) hipStreamDestroy(ln);
-  if (b->master) hipStreamDestroy(b->master);
+  for (hipStream_t st : {b->side, b->master})
+    if (st) hipStreamDestroy(st);
   delete b;
   return DM_OK;
 }
 
 int dm_batch_create(int dtype, int ncirc, const int *n_qubits, const int64_t *op_start, const dm_op *ops,
                     const double *params, int64_t nparams, const double *meas_err, int lds_max_qubits,
-                    int lds_threads, dm_batch **out) {
+                    int lds_threads, int items_per_pass, dm_batch **out) {
   if (!out) return fail(DM_ERR_ARG, "null output pointer");
   *out = nullptr;
   if (dtype != DM_FP64 && dtype != DM_FP32) return fail(DM_ERR_ARG, "unknown dtype %d", dtype);
   if (ncirc < 1 || !n_qubits || !op_start || !meas_err) return fail(DM_ERR_ARG, "bad circuit list");
   if (lds_threads != 256 && lds_threads != 512 && lds_threads != 1024)
     return fail(DM_ERR_ARG, "lds_threads must be 256, 512 or 1024");
-  if (lds_max_qubits > 7) return fail(DM_ERR_ARG, "LDS circuits hold at most 7 qubits");
-  if (dtype == DM_FP64 && lds_max_qubits > 7) return fail(DM_ERR_ARG, "bad LDS width");
-  {
-    // circuits that keep their state in HBM, counted before anything is
-    // allocated (each has its own state, program and per-pass launches)
-    int hbm = 0;
-    for (int c = 0; c < ncirc; ++c) hbm += n_qubits[c] > lds_max_qubits ? 1 : 0;
-    if (hbm > kBatchMaxHbmCircuits)
-      return fail(DM_ERR_ARG, "batch holds %d circuits above the LDS width %d; at most %d are supported", hbm,
-                  lds_max_qubits, kBatchMaxHbmCircuits);
+  if (lds_max_qubits < kQueueTile - 1 || lds_max_qubits > 7)
+    return fail(DM_ERR_ARG, "lds_max_qubits must lie in [%d, 7]", kQueueTile - 1);
+  if (items_per_pass < 1 || items_per_pass > 65536) return fail(DM_ERR_ARG, "items_per_pass must lie in [1, 65536]");
+  if (nparams < 0 || (nparams > 0 && !params)) return fail(DM_ERR_ARG, "bad parameter array");
+  for (int c = 0; c < ncirc; ++c) {
+    if (n_qubits[c] < 1 || n_qubits[c] > kQueueMaxQubits)
+      return fail(DM_ERR_ARG, "circuit %d: %d qubits outside [1, %d]", c, n_qubits[c], kQueueMaxQubits);
+    if (op_start[c + 1] < op_start[c] || (op_start[c + 1] > op_start[c] && !ops))
+      return fail(DM_ERR_ARG, "circuit %d: bad op range", c);
   }
   dm_batch *b = new dm_batch();
   b->dtype = dtype;
   b->ncirc = ncirc;
   b->lds_threads = lds_threads;
   for (int c = 0; c < ncirc; ++c) {
-    if (n_qubits[c] < 1 || n_qubits[c] > 16) {
-      dm_batch_destroy(b);
-      return fail(DM_ERR_ARG, "circuit %d: %d qubits outside [1, 16]", c, n_qubits[c]);
-    }
-    if (op_start[c + 1] < op_start[c]) {
-      dm_batch_destroy(b);
-      return fail(DM_ERR_ARG, "circuit %d: bad op range", c);
-    }
     b->nq.push_back(n_qubits[c]);
     b->prob_off.push_back(b->nprobs);
     b->nprobs += (int64_t)1 << n_qubits[c];
     b->err.push_back(meas_err[c]);
   }
-  const int rc = dtype == DM_FP64 ? batch_create_t<double>(b, op_start, ops, params, nparams, lds_max_qubits)
-                                  : batch_create_t<float>(b, op_start, ops, params, nparams, lds_max_qubits);
+  const int rc = dtype == DM_FP64
+                     ? batch_create_t<double>(b, op_start, ops, params, nparams, lds_max_qubits, items_per_pass)
+                     : batch_create_t<float>(b, op_start, ops, params, nparams, lds_max_qubits, items_per_pass);
   if (rc) {
     dm_batch_destroy(b);
     return rc;
@@ -2978,11 +3282,17 @@ int dm_batch_create(int dtype, int ncirc, const int *n_qubits, const int64_t *op
   return DM_OK;
 }
 
+int dm_batch_info(const dm_batch *b, int64_t *hbm_circuits, int64_t *passes, int64_t *items, int64_t *launches) {
+  if (!b) return fail(DM_ERR_ARG, "null batch");
+  if (hbm_circuits) *hbm_circuits = (int64_t)b->large.size();
+  if (passes) *passes = b->npasses;
+  if (items) *items = b->nitems;
+  if (launches) *launches = b->launches;
+  return DM_OK;
+}
+
 int dm_batch_run(dm_batch *b, int use_graph, double *probs_out, double *circuit_us, double *total_ms) {
   if (!b) return fail(DM_ERR_ARG, "null batch");
-  if (b->launches > kBatchGraphLaunchLimit)
-    return fail(DM_ERR_ARG, "batch run of %lld kernel launches exceeds the limit (%lld)",
-                (long long)b->launches, (long long)kBatchGraphLaunchLimit);
   if (use_graph && !b->exec) {
     HIP_TRY(hipStreamBeginCapture(b->master, hipStreamCaptureModeThreadLocal));
     const int rc = b->dtype == DM_FP64 ? enqueue_batch<double>(b) : enqueue_batch<float>(b);
@@ -3012,7 +3322,7 @@ int dm_batch_run(dm_batch *b, int use_graph, double *probs_out, double *circuit_
   if (circuit_us) {
     std::vector<uint64_t> st(2 * (size_t)b->ncirc);
     HIP_TRY(hipMemcpy(st.data(), b->d_stamps, st.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
-    // stamps are stored small circuits first, then HBM circuits; report in circuit order
+    // stamps are stored LDS-tier circuits first, then HBM-tier circuits; report in circuit order
     std::vector<int> order(b->small);
     order.insert(order.end(), b->large.begin(), b->large.end());
     for (size_t j = 0; j < order.size(); ++j)

@@ -13,6 +13,9 @@ HIPCC_FLAGS = [
     # packed FP32 (v_pk_*_f32) stays off for every HIP translation unit
     '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
     '-Wno-unused-result',
+    # no FMA contraction in host code: the planner's scores and the host MLP
+    # forward round the same way wherever they are evaluated
+    '-ffp-contract=off',
 ]
 
 
@@ -24,8 +27,9 @@ def hipcc():
 
 
 def build(verbose=False):
-    """Compile csrc/*.hip into dmsim/libdmsim.so (always recompiles)."""
-    srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith('.hip'))
+    """Compile csrc/*.hip (device + host) and csrc/*.cpp (host) into
+    dmsim/libdmsim.so (always recompiles)."""
+    srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(('.hip', '.cpp')))
     tmp = LIB + '.tmp'
     cmd = [hipcc()] + HIPCC_FLAGS + ['-I', INCLUDE] + srcs + ['-o', tmp]
     if verbose:

@@ -36,6 +36,47 @@ logger = logging.getLogger(__name__)
 
 # device staging buffer of the dense output's scatter (dm_to_dense)
 DENSE_CHUNK_BYTES = 1 << 31
+# coefficients per chunk of the streamed .npy state files (512 MB in FP64)
+IO_CHUNK = 1 << 26
+
+
+def store_state(st, path):
+    """Write the state's coefficients to the .npy file ``path`` in chunks
+    (the reference's _store_density_matrix, dm_simulator.py:1271-1275, holds
+    the whole FP64 array).  The file takes the storage precision (float64 or
+    float32); it is written under a temporary name and renamed, so a reader
+    that mapped the previous file keeps its contents."""
+    dtype = np.float32 if st.dtype == native.FP32 else np.float64
+    tmp = path[:-4] + '.partial.npy'
+    mm = np.lib.format.open_memmap(tmp, mode='w+', dtype=dtype, shape=(st.size,))
+    try:
+        for i in range(0, st.size, IO_CHUNK):
+            st.download_range(i, mm[i:i + IO_CHUNK])
+        mm.flush()
+    finally:
+        del mm
+    os.replace(tmp, path)
+
+
+def load_state(path):
+    """The coefficients of a .npy state file, memory-mapped (not read)."""
+    a = np.load(path, mmap_mode='r')
+    if a.ndim != 1 or a.dtype not in (np.float64, np.float32):
+        a = np.asarray(a, dtype=np.float64).reshape(-1)
+    return a
+
+
+def upload_state(st, a):
+    for i in range(0, a.size, IO_CHUNK):
+        st.upload_range(i, np.ascontiguousarray(a[i:i + IO_CHUNK]))
+
+
+def state_overlap(st, a):
+    """2^n <a_state, a> (_state_overlap, dm_simulator.py:1277-1282), chunked."""
+    total = 0.0
+    for i in range(0, a.size, IO_CHUNK):
+        total += st.dot_range(i, np.ascontiguousarray(a[i:i + IO_CHUNK]))
+    return total * 2.0 ** st.n
 
 
 def _host_available_bytes():
@@ -259,7 +300,7 @@ class DmSimulator:
         if 'compare' in opts:
             self.COMPARE = opts['compare']
             try:
-                self._density_matrix_stored = np.load('stored_coefficients.npy')
+                self._density_matrix_stored = load_state('stored_coefficients.npy')
                 self.FILE_EXIST = True
             except FileNotFoundError:
                 print('Stored Coefficient File does not exist')
@@ -295,7 +336,9 @@ class DmSimulator:
         snap = copy.copy(self)
         for attr in ('_rotation_error', '_tsp_model_error', '_initial_densitymatrix', '_density_matrix_stored'):
             val = getattr(self, attr, None)
-            if isinstance(val, (dict, list, np.ndarray)):
+            # a memory-mapped stored state is read-only and outlives the
+            # rename of a newer file: shared, not copied
+            if isinstance(val, (dict, list, np.ndarray)) and not isinstance(val, np.memmap):
                 setattr(snap, attr, copy.deepcopy(val))
         return snap
 
@@ -378,14 +421,14 @@ class DmSimulator:
             return
         if custom == 'stored_density_matrix':
             try:
-                a = np.load('stored_density_matrix.npy')
+                a = load_state('stored_density_matrix.npy')
             except FileNotFoundError:
                 raise BasicAerError('Stored Coefficient File does not exist')
             if len(a) != 4 ** n:
                 raise BasicAerError('Wrong input stored density matrix')
             if a[0] != 2.0 ** (-n):
                 raise BasicAerError('Trace of initial densitymatrix is not one: {} != {}'.format(a[0], 2.0 ** -n))
-            st.upload(a)
+            upload_state(st, a)
             return
         raise BasicAerError('_custom_densitymatrix value is invalid')
 
@@ -422,9 +465,11 @@ class DmSimulator:
         vals = st.probabilities(qs, basis, err, nv)
         keys = [''.join(s) for s in itertools.product('01', repeat=len(qs))]
         if self.STORE_LOCAL:
-            np.save('stored_coefficients', st.download())
+            store_state(st, 'stored_coefficients.npy')
         if self.COMPARE and self.FILE_EXIST:
-            self._fidelity = st.overlap(self._density_matrix_stored)
+            if len(self._density_matrix_stored) != st.size:
+                raise BasicAerError('Stored coefficients of a different register size')
+            self._fidelity = state_overlap(st, self._density_matrix_stored)
         return dict(zip(keys, [float(v) for v in vals]))
 
     def _bell(self, st, n, qa, qb, err):

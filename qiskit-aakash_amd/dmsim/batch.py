@@ -5,13 +5,15 @@ after every gate: the gate, then memory noise on every qubit (one level per
 gate, dm_simulator.py:1172-1177), and finally an ensemble Z measurement of
 all qubits with polarisation retention ``depolarization_factor``.  Small
 registers (<= ``lds_max_qubits``) run whole inside one workgroup's LDS; the
-others keep their state in HBM and run on parallel streams, as fused tile
-programs from 6 qubits up; the batch is captured once into a hipGraph and
-replayed (``dm_batch_*`` in dmsim.h).
+others keep their state in one HBM arena and run as fused 6-digit tile
+passes, one launch per round (pass r of every circuit), each pass split into
+at most ``items_per_pass`` work items; the batch is captured once into a
+hipGraph and replayed (``dm_batch_*`` in dmsim.h).
 
-``autotune`` picks the LDS cutoff width and the workgroup size by timing the
-candidates on the batch itself and caches the choice per (precision, CU
-count, circuit count, width range).
+``autotune`` picks the LDS cutoff width, the LDS workgroup size and the
+HBM tier's items per pass by timing the candidates on the batch itself; the
+choice is kept per (precision, CU count, circuit count, width range) for the
+process, and in the JSON file named by DMSIM_TUNE_CACHE when that is set.
 """
 import ctypes
 import itertools
@@ -23,12 +25,14 @@ import numpy as np
 from . import native, ptm
 from .program import Program
 
-_TUNE_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'batch_tuning.json')
+_TUNED = {}
 
-# circuits above the LDS cutoff keep their state in HBM; a graph-captured run
-# holds at most 16384 launches (dm_batch_run), and large HBM-tier batches stay
-# off until the failure of a 512-circuit run is understood
-MAX_HBM_CIRCUITS = 12
+# tile digits of the HBM tier's passes (dm_batch_create takes 6-digit fused tiles)
+QUEUE_TILE = 6
+# autotuner candidates
+LDS_MAX_CHOICES = (5, 6, 7)
+LDS_THREAD_CHOICES = (256, 512, 1024)
+ITEMS_PER_PASS_CHOICES = (64, 256, 1024)
 
 
 def circuit_program(n, gates, decay_factor=1.0, decoherence_factor=1.0, thermal_factor=1.0,
@@ -54,7 +58,8 @@ def fused_circuit_program(n, gates, decay_factor=1.0, decoherence_factor=1.0, th
     """The same circuit as ``circuit_program`` as fused tile passes: the
     memory noise of every level enters the gate stream as one 1-qubit
     channel per qubit, and the greedy planner groups everything into blocks
-    of at most 4 qubits (registers of at least 6 qubits)."""
+    of at most 4 qubits, each pass a 6-digit tile (registers of at least 6
+    qubits)."""
     from . import fusion
     noisy = []
     mem = ptm.memory_noise(decoherence_factor, thermal_factor, decay_factor)
@@ -63,14 +68,14 @@ def fused_circuit_program(n, gates, decay_factor=1.0, decoherence_factor=1.0, th
         noisy += fusion.noisy_gates([gate], gate_depolarization)
         if not identity:
             noisy += [fusion.NoisyGate((q,), mem, 'u3') for q in range(n)]
-    prog, _ = fusion.fused_program(noisy, n, fusion.KMAX['greedy'], 'greedy', None, 'seq')
+    prog, _ = fusion.fused_program(noisy, n, fusion.KMAX['greedy'], 'greedy', None, 'seq', tile=QUEUE_TILE)
     return prog
 
 
 class Batch:
     """A compiled batch on the device."""
 
-    def __init__(self, circuits, precision='fp64', lds_max_qubits=7, lds_threads=256):
+    def __init__(self, circuits, precision='fp64', lds_max_qubits=7, lds_threads=256, items_per_pass=256):
         """``circuits``: list of (n, Program, meas_err)."""
         self.lib = native.load()
         self.ns = [int(c[0]) for c in circuits]
@@ -97,9 +102,15 @@ class Batch:
             self._n.ctypes.data_as(native.c_int_p), self._starts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
             ctypes.cast(self._ops, ctypes.c_void_p), self._params.ctypes.data_as(native.c_double_p),
             int(off), self._err.ctypes.data_as(native.c_double_p), int(lds_max_qubits), int(lds_threads),
-            ctypes.byref(h)), 'dm_batch_create')
+            int(items_per_pass), ctypes.byref(h)), 'dm_batch_create')
         self.h = h
         self.nprobs = sum(2 ** n for n in self.ns)
+
+    def info(self):
+        """HBM-tier circuits, passes and work items, kernel launches per run."""
+        v = [ctypes.c_int64(0) for _ in range(4)]
+        native.check(self.lib.dm_batch_info(self.h, *[ctypes.byref(x) for x in v]), 'dm_batch_info')
+        return dict(zip(('hbm_circuits', 'passes', 'items', 'launches'), (x.value for x in v)))
 
     def run(self, use_graph=True):
         """Returns (list of probability vectors, per-circuit GPU microseconds,
@@ -130,45 +141,63 @@ class Batch:
 def from_workload(specs, precision='fp64', lds_max_qubits=7, **kw):
     """Batch from dmsim.workloads.batch specs (n, gates, decay_rate, depol_rate):
     decay_factor = 1 - decay_rate, measurement retention = 1 - depol_rate.
-    Circuits above the LDS cutoff (and of at least 6 qubits) run as fused
-    tile passes."""
+    Circuits above the LDS cutoff run as fused tile passes."""
     circuits = []
     for n, gates, dr, pr in specs:
-        build = fused_circuit_program if (n > lds_max_qubits and n >= 6) else circuit_program
+        build = fused_circuit_program if n > lds_max_qubits else circuit_program
         circuits.append((n, build(n, gates, decay_factor=1.0 - dr), 1.0 - pr))
     return Batch(circuits, precision, lds_max_qubits=lds_max_qubits, **kw)
 
 
+def _tune_key(specs, precision):
+    ns = [sp[0] for sp in specs]
+    return '%s:%s:%d:%d-%d' % (precision, native.device_info(0)['cu_count'], len(specs), min(ns), max(ns))
+
+
 def autotune(specs, precision='fp64', reps=2, use_cache=True):
-    """Best (lds_max_qubits, lds_threads) for this batch by measured time."""
-    ns = [sp[0] for sp in specs]
-    key = '%s:%s:%d:%d-%d' % (precision, native.device_info(0)['cu_count'], len(specs), min(ns), max(ns))
-    cache = {}
-    if use_cache and os.path.exists(_TUNE_CACHE):
-        try:
-            cache = json.load(open(_TUNE_CACHE))
-        except (ValueError, OSError):
-            cache = {}
-        if key in cache:
-            return tuple(cache[key]['choice']), cache[key]
+    """Best (lds_max_qubits, lds_threads, items_per_pass) for this batch by
+    measured time: the LDS cutoff and workgroup size first (items per pass
+    256), then the items per pass at that cutoff."""
+    key = _tune_key(specs, precision)
+    path = os.environ.get('DMSIM_TUNE_CACHE')
+    if use_cache:
+        if key in _TUNED:
+            return tuple(_TUNED[key]['choice']), _TUNED[key]
+        if path and os.path.exists(path):
+            try:
+                with open(path) as f:
+                    disk = json.load(f)
+                if key in disk:
+                    _TUNED[key] = disk[key]
+                    return tuple(disk[key]['choice']), disk[key]
+            except (ValueError, OSError):
+                pass
     trials = {}
-    ns = [sp[0] for sp in specs]
-    # a cutoff that leaves more than MAX_HBM_CIRCUITS circuits in HBM is not
-    # a candidate (their per-operation launches exceed the graph limit)
-    lmaxes = [l for l in (5, 6, 7) if sum(1 for n in ns if n > l) <= MAX_HBM_CIRCUITS] or [7]
-    for lmax, thr in itertools.product(lmaxes, (256, 512, 1024)):
-        b = from_workload(specs, precision, lds_max_qubits=lmax, lds_threads=thr)
+
+    def trial(lmax, thr, ipp):
+        b = from_workload(specs, precision, lds_max_qubits=lmax, lds_threads=thr, items_per_pass=ipp)
         b.run(True)
         t = min(b.run(True)[2] for _ in range(reps))
         b.close()
-        trials['%d/%d' % (lmax, thr)] = t
-    best = min(trials, key=trials.get)
-    choice = tuple(int(x) for x in best.split('/'))
+        trials['%d/%d/%d' % (lmax, thr, ipp)] = t
+        return t
+
+    best = min(((lmax, thr) for lmax, thr in itertools.product(LDS_MAX_CHOICES, LDS_THREAD_CHOICES)),
+               key=lambda c: trial(c[0], c[1], 256))
+    ipp = min(ITEMS_PER_PASS_CHOICES, key=lambda i: trials.get('%d/%d/%d' % (best + (i,))) or trial(*best, i))
+    choice = best + (ipp,)
     rec = {'choice': list(choice), 'trials_ms': trials}
     if use_cache:
-        cache[key] = rec
-        try:
-            json.dump(cache, open(_TUNE_CACHE, 'w'), indent=1)
-        except OSError:
-            pass
+        _TUNED[key] = rec
+        if path:
+            try:
+                disk = {}
+                if os.path.exists(path):
+                    with open(path) as f:
+                        disk = json.load(f)
+                disk[key] = rec
+                with open(path, 'w') as f:
+                    json.dump(disk, f, indent=1)
+            except (ValueError, OSError):
+                pass
     return choice, rec

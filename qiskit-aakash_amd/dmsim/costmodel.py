@@ -114,10 +114,13 @@ class CostModel:
         np.savez(path, params=self.params, meta=json.dumps(self.meta))
 
     def predict_many(self, rows, n, precision):
-        """Seconds for each (kind, qubits, n1, n2[, n2low, d2]) row."""
+        """Seconds for each (kind, qubits, n1, n2[, n2low, d2]) row: the host
+        forward pass and the C library's exp, as the native planner
+        evaluates them (dm_plan_fusion), so both plan alike."""
         X = np.stack([features(r[0], n, r[1], r[2], r[3], precision, *r[4:]) for r in rows])
-        y = native.mlp_predict(self.params, N_FEATURES, HIDDEN, X)
-        return np.exp(y.astype(np.float64)) * state_gib(n, precision)
+        y = native.mlp_predict_host(self.params, N_FEATURES, HIDDEN, X)
+        gib = state_gib(n, precision)
+        return np.array([math.exp(float(v)) * gib for v in y])
 
     def _time(self, kind, n, qs, st):
         key = (kind, n, tuple(sorted(qs))) + tuple(st) + (self.precision,)

@@ -336,10 +336,22 @@ def _lookahead_pick(gates, done, first, cands, score, per_gate, widths, ctx):
 
 
 def plan(gates, n, kmax=4, method='greedy', cost=None, tiles=True, lookahead=LOOKAHEAD):
-    """Blocks as (qubits, [gate indices]) in execution order.  With ``tiles``
+    """Blocks as (qubits, [gate indices]) in execution order, from the native
+    planner (dm_plan_fusion, the same rules as ``plan_py``).  With ``tiles``
     (blocks run as sequential tile passes of the default size) the learned
     planner's candidates include the tile-wide blocks (``candidate_blocks``);
     ``lookahead`` gives its search widths (LOOKAHEAD), () none."""
+    if method not in native.PLAN_METHODS:
+        raise ValueError('unknown fusion method %r' % method)
+    gq = [(g.qubits[0], g.qubits[1] if len(g.qubits) == 2 else -1) for g in gates]
+    if method == 'learned':
+        return native.plan_fusion(n, gq, method, kmax, tiles, tuple(lookahead), cost.params, cost.precision)
+    return native.plan_fusion(n, gq, method, kmax, tiles)
+
+
+def plan_py(gates, n, kmax=4, method='greedy', cost=None, tiles=True, lookahead=LOOKAHEAD):
+    """The planner in Python (the cross-check of the native one): blocks as
+    (qubits, [gate indices]) in execution order."""
     done = [False] * len(gates)
     blocks = []
     first = 0

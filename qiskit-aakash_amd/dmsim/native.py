@@ -49,6 +49,22 @@ SIGNATURES = {
     'dm_to_dense': (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
     'dm_compare': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_double_p, c_double_p, c_double_p,
                                   c_double_p]),
+    'dm_download_range': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                         ctypes.c_void_p]),
+    'dm_upload_range': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                       ctypes.c_void_p]),
+    'dm_dot_range': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                    ctypes.c_void_p, c_double_p]),
+    'dm_merge_rotations': (ctypes.c_int, [ctypes.c_int, c_int_p, c_int_p, c_double_p, ctypes.c_int, c_int_p, c_int_p,
+                                          c_double_p, c_int_p]),
+    'dm_partition': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p,
+                                    c_int_p]),
+    'dm_plan_fusion': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, c_int_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      c_int_p, ctypes.c_int, c_float_p, ctypes.c_int, c_int_p, c_int_p, c_int_p,
+                                      c_int_p, c_int_p]),
+    'dm_mlp_predict_host': (ctypes.c_int, [c_float_p, ctypes.c_int, ctypes.c_int, c_float_p, ctypes.c_int, c_float_p]),
+    'dm_cost_block_seconds': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, c_int_p, ctypes.c_int, c_int_p, c_int_p,
+                                             c_int_p, c_int_p, c_float_p, ctypes.c_int, c_double_p]),
     'dm_run_program': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, c_double_p,
                                       ctypes.c_int64]),
     'dm_mlp_param_count': (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
@@ -60,7 +76,9 @@ SIGNATURES = {
                                     ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, c_float_p]),
     'dm_batch_create': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, c_int_p, ctypes.POINTER(ctypes.c_int64),
                                        ctypes.c_void_p, c_double_p, ctypes.c_int64, c_double_p, ctypes.c_int,
-                                       ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+                                       ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    'dm_batch_info': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                     ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     'dm_batch_run': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_double_p, c_double_p, c_double_p]),
     'dm_batch_destroy': (ctypes.c_int, [ctypes.c_void_p]),
     'dm_synchronize': (ctypes.c_int, [ctypes.c_void_p]),
@@ -190,6 +208,34 @@ class DeviceState:
         else:
             check(self.lib.dm_download(self.h, _dp(out)), 'dm_download')
         return out
+
+    def _host_range(self, arr, offset):
+        a = np.asarray(arr)
+        if a.dtype not in (np.float64, np.float32) or a.ndim != 1 or not a.flags.c_contiguous:
+            raise ValueError('host range must be a contiguous 1-d float64 or float32 array')
+        return a, (FP64 if a.dtype == np.float64 else FP32), ctypes.c_uint64(int(offset)), ctypes.c_uint64(a.size)
+
+    def download_range(self, offset, out):
+        """Coefficients [offset, offset + out.size) into ``out`` (float64 or
+        float32, converted on the device)."""
+        a, hd, off, cnt = self._host_range(out, offset)
+        if not a.flags.writeable:
+            raise ValueError('output array is read-only')
+        check(self.lib.dm_download_range(self.h, off, cnt, hd, a.ctypes.data_as(ctypes.c_void_p)),
+              'dm_download_range')
+        return out
+
+    def upload_range(self, offset, values):
+        a, hd, off, cnt = self._host_range(values, offset)
+        check(self.lib.dm_upload_range(self.h, off, cnt, hd, a.ctypes.data_as(ctypes.c_void_p)), 'dm_upload_range')
+
+    def dot_range(self, offset, values):
+        """FP64 sum of a_i * values[i - offset] over the range."""
+        a, hd, off, cnt = self._host_range(values, offset)
+        out = ctypes.c_double(0)
+        check(self.lib.dm_dot_range(self.h, off, cnt, hd, a.ctypes.data_as(ctypes.c_void_p), ctypes.byref(out)),
+              'dm_dot_range')
+        return out.value
 
     def read(self, indices):
         idx = np.ascontiguousarray(np.asarray(indices, dtype=np.uint64))
@@ -337,3 +383,91 @@ def mlp_train(params, n_features, hidden, X, y, epochs, batch, lr, seed):
     check(load().dm_mlp_train(_fp(p), int(n_features), int(hidden), _fp(X), _fp(y), y.size, int(epochs),
                               int(batch), float(lr), int(seed), _fp(losses)), 'dm_mlp_train')
     return p, losses[:epochs]
+
+
+def mlp_predict_host(params, n_features, hidden, X):
+    """The perceptron's forward pass on the host (the planner's predictions)."""
+    X = _f32(X)
+    n = X.shape[0] if X.ndim == 2 else 0
+    out = np.zeros(max(n, 1), dtype=np.float32)
+    check(load().dm_mlp_predict_host(_fp(_f32(params)), int(n_features), int(hidden), _fp(X), n, _fp(out)),
+          'dm_mlp_predict_host')
+    return out[:n]
+
+
+# ---------------------------------------------------------------------------
+# host front end: merge, partition, fusion planning
+# ---------------------------------------------------------------------------
+
+INST_U1, INST_U3, INST_ID, INST_SEPARATOR = 1, 2, 3, 4
+INST_ROT, INST_CX, INST_MEASURE, INST_RESET, INST_BARRIER, INST_OWN_LEVEL = 5, 6, 7, 8, 9, 10
+PLAN_METHODS = {'greedy': 0, 'search': 1, 'learned': 2}
+
+
+def _i32(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.int32))
+
+
+def _ip(a):
+    return a.ctypes.data_as(c_int_p)
+
+
+def merge_rotations(kind, qubit, params, merge=True):
+    """(src, kind, params[k, 3]) of the merged instruction list."""
+    kind, qubit = _i32(kind), _i32(qubit)
+    p = _f64(np.asarray(params, dtype=np.float64).reshape(-1))
+    m = kind.size
+    src = np.zeros(max(m, 1), dtype=np.int32)
+    okind = np.zeros(max(m, 1), dtype=np.int32)
+    op = np.zeros(3 * max(m, 1))
+    nout = ctypes.c_int(0)
+    check(load().dm_merge_rotations(m, _ip(kind), _ip(qubit), _dp(p), 1 if merge else 0, _ip(src), _ip(okind),
+                                    _dp(op), ctypes.byref(nout)), 'dm_merge_rotations')
+    k = nout.value
+    return src[:k], okind[:k], op[:3 * k].reshape(k, 3)
+
+
+def partition_levels(n, kind, q0, q1):
+    """(levels as lists of instruction indices, count)."""
+    kind, q0, q1 = _i32(kind), _i32(q0), _i32(q1)
+    m = kind.size
+    ops = np.zeros(max(m, 1), dtype=np.int32)
+    starts = np.zeros(4 * m + 4, dtype=np.int32)
+    nl, cnt = ctypes.c_int(0), ctypes.c_int(0)
+    check(load().dm_partition(int(n), m, _ip(kind), _ip(q0), _ip(q1), _ip(ops), _ip(starts), ctypes.byref(nl),
+                              ctypes.byref(cnt)), 'dm_partition')
+    levels = [ops[starts[i]:starts[i + 1]].tolist() for i in range(nl.value)]
+    return levels, cnt.value
+
+
+def plan_fusion(n, gate_qubits, method, kmax, tiles, lookahead=(), params=None, precision='fp32'):
+    """Blocks (qubits, members) of the fusion planner."""
+    gq = _i32(gate_qubits).reshape(-1)
+    ng = gq.size // 2
+    la = _i32(list(lookahead) or [0])
+    p = _f32(params) if params is not None else None
+    members = np.zeros(max(ng, 1), dtype=np.int32)
+    bstart = np.zeros(ng + 1, dtype=np.int32)
+    qubits = np.zeros(max(2 * ng, 1), dtype=np.int32)
+    qstart = np.zeros(ng + 1, dtype=np.int32)
+    nb = ctypes.c_int(0)
+    check(load().dm_plan_fusion(int(n), ng, _ip(gq), PLAN_METHODS[method], int(kmax), 1 if tiles else 0, _ip(la),
+                                len(lookahead), _fp(p) if p is not None else None,
+                                FP64 if precision == 'fp64' else FP32, _ip(members), _ip(bstart), _ip(qubits),
+                                _ip(qstart), ctypes.byref(nb)), 'dm_plan_fusion')
+    return [(qubits[qstart[b]:qstart[b + 1]].tolist(), members[bstart[b]:bstart[b + 1]].tolist())
+            for b in range(nb.value)]
+
+
+def cost_block_seconds(n, gate_qubits, blocks, params, precision='fp32'):
+    """The cost model's predicted seconds of each (qubits, members) block."""
+    gq = _i32(gate_qubits).reshape(-1)
+    members = _i32([i for _, mem in blocks for i in mem] or [0])
+    qubits = _i32([q for qs, _ in blocks for q in qs] or [0])
+    bstart = _i32(np.cumsum([0] + [len(mem) for _, mem in blocks]))
+    qstart = _i32(np.cumsum([0] + [len(qs) for qs, _ in blocks]))
+    out = np.zeros(max(len(blocks), 1))
+    check(load().dm_cost_block_seconds(int(n), gq.size // 2, _ip(gq), len(blocks), _ip(members), _ip(bstart),
+                                       _ip(qubits), _ip(qstart), _fp(_f32(params)),
+                                       FP64 if precision == 'fp64' else FP32, _dp(out)), 'dm_cost_block_seconds')
+    return out[:len(blocks)]

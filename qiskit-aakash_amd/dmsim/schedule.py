@@ -1,7 +1,10 @@
 """Host scheduling: single-qubit gate merging and level partitioning.
 
 The level structure decides how often the per-level memory noise is applied,
-so both passes keep the reference's semantics exactly:
+so both passes keep the reference's semantics exactly.  ``merge_single_gates``
+and ``partition`` run the native front end (dm_merge_rotations,
+dm_partition in libdmsim); ``merge_single_gates_py`` and ``partition_py``
+are the same passes in Python, kept as the cross-check of the native ones:
 
 * ``merge_single_gates`` -- basicaertools.py:129-307 (U3_merge, mergeU,
   merge_gates, single_gate_merge).  Runs of u1/u2/u3 on a qubit between
@@ -93,7 +96,55 @@ def _normalise(op):
         op.params.insert(0, np.pi / 2)
 
 
+def _merge_kind(op):
+    if op.name in ROTATIONS:
+        return 0
+    if op.name in SEPARATORS:
+        return 1
+    if op.name in ('id', 'u0'):
+        return 2
+    raise BasicAerError('Encountered unrecognized instruction: %s' % op.name)
+
+
 def merge_single_gates(insts, n, merge=True):
+    """Merged instruction list through the native front end: each output
+    rotation is a copy of the first record of its run with the folded name
+    and angles (merged angles from the C library's trigonometry, within a
+    few ulp of numpy's); separators are the input records, normalised.
+    Records are normalised in place, as in the reference (pass copies)."""
+    from . import native
+    kinds = [_merge_kind(op) for op in insts]
+    for op, k in zip(insts, kinds):
+        if k == 0:
+            _normalise(op)
+    kc = [(native.INST_U1 if op.name == 'u1' else native.INST_U3) if k == 0 else
+          (native.INST_SEPARATOR if k == 1 else native.INST_ID) for op, k in zip(insts, kinds)]
+    qs = [op.qubits[0] if k == 0 else 0 for op, k in zip(insts, kinds)]
+    prm = np.zeros((len(insts), 3))
+    for i, (op, k) in enumerate(zip(insts, kinds)):
+        if k == 0:
+            v = [float(x) for x in op.params]
+            prm[i, :len(v)] = v
+    src, okind, oprm = native.merge_rotations(kc, qs, prm, merge)
+    out = []
+    for j, i in enumerate(src):
+        op = insts[i]
+        if okind[j] == native.INST_SEPARATOR:
+            _normalise(op)
+            out.append(op)
+            continue
+        rec = copy.deepcopy(op)
+        if okind[j] == native.INST_U1:
+            rec.name = 'u1'
+            rec.params = [float(oprm[j, 0])]
+        else:
+            rec.name = 'u3'
+            rec.params = [float(x) for x in oprm[j]]
+        out.append(rec)
+    return out
+
+
+def merge_single_gates_py(insts, n, merge=True):
     """Merged instruction list (records are modified in place, as in the
     reference; pass copies)."""
     out = []
@@ -223,7 +274,41 @@ def _level_segment(seg, n):
     return levels, level
 
 
+def _partition_kind(op, n):
+    from . import native
+    if op.name == 'bfunc' or not hasattr(op, 'qubits') and op.name != 'barrier':
+        raise BasicAerError('dm_simulator cannot partition "%s": conditional operations are not '
+                            'supported on this backend' % op.name)
+    if op.name == 'barrier':
+        return native.INST_BARRIER
+    if op.name in ('u1', 'u3'):
+        return native.INST_ROT
+    if op.name in ('cx', 'CX'):
+        return native.INST_CX
+    if op.name == 'reset':
+        return native.INST_RESET
+    if op.name == 'measure':
+        prm = getattr(op, 'params', None)
+        return native.INST_OWN_LEVEL if prm is not None and str(prm[0]) in OWN_LEVEL else native.INST_MEASURE
+    raise BasicAerError('dm_simulator encountered unrecognized operation "%s"' % op.name)
+
+
 def partition(insts, n):
+    """Split at barriers and level each segment (native front end).
+    Returns (levels, count); the clock loop runs ``count`` levels over
+    ``levels`` (which may carry the reference's trailing empty lists)."""
+    from . import native
+    kinds = [_partition_kind(op, n) for op in insts]
+    q0 = [op.qubits[0] if getattr(op, 'qubits', None) else 0 for op in insts]
+    q1 = [op.qubits[1] if k == native.INST_CX else -1 for op, k in zip(insts, kinds)]
+    try:
+        lv, count = native.partition_levels(n, kinds, q0, q1)
+    except native.NativeError as e:
+        raise BasicAerError(str(e).split(': ', 1)[-1])
+    return [[insts[i] for i in level] for level in lv], count
+
+
+def partition_py(insts, n):
     """Split at barriers and level each segment.  Returns (levels, count);
     the clock loop runs ``count`` levels over ``levels`` (which may carry the
     reference's trailing empty lists)."""

@@ -114,16 +114,26 @@ def test_reference_provider_registration_resolves_the_gpu_backend():
     assert isinstance(get_backend('qasm_simulator'), RefQasmSimulatorPy)
 
 
-def test_batch_create_refuses_large_hbm_tier_before_allocating():
+def test_batch_create_checks_arguments_before_allocating():
     lib = native.load()
-    n = 13
-    nq = (ctypes.c_int * n)(*([8] * n))
-    starts = (ctypes.c_int64 * (n + 1))(*([0] * (n + 1)))
-    err = (ctypes.c_double * n)(*([1.0] * n))
-    ops = (native.DmOp * 1)()
-    params = (ctypes.c_double * 1)()
-    h = ctypes.c_void_p()
-    rc = lib.dm_batch_create(native.FP64, n, nq, starts, ctypes.cast(ops, ctypes.c_void_p), params, 0, err, 7,
-                             256, ctypes.byref(h))
-    assert rc == -1 and not h.value
-    assert 'at most' in native.last_error()
+
+    def create(widths, lds_max=7, threads=256, ipp=256):
+        n = len(widths)
+        nq = (ctypes.c_int * n)(*widths)
+        starts = (ctypes.c_int64 * (n + 1))(*([0] * (n + 1)))
+        err = (ctypes.c_double * n)(*([1.0] * n))
+        ops = (native.DmOp * 1)()
+        params = (ctypes.c_double * 1)()
+        h = ctypes.c_void_p()
+        rc = lib.dm_batch_create(native.FP64, n, nq, starts, ctypes.cast(ops, ctypes.c_void_p), params, 0, err,
+                                 lds_max, threads, ipp, ctypes.byref(h))
+        return rc, h.value, native.last_error()
+
+    rc, h, msg = create([8, 13])
+    assert rc == -1 and not h and '13 qubits' in msg
+    rc, h, msg = create([8], lds_max=4)
+    assert rc == -1 and not h and 'lds_max_qubits' in msg
+    rc, h, msg = create([8], ipp=0)
+    assert rc == -1 and not h and 'items_per_pass' in msg
+    rc, h, msg = create([8], threads=100)
+    assert rc == -1 and not h and 'lds_threads' in msg

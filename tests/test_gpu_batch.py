@@ -1,5 +1,6 @@
-"""Batched execution (LDS-resident small circuits + HBM circuits in one
-hipGraph) against the CPU oracle, circuit by circuit."""
+"""Batched execution (LDS-resident small circuits + HBM circuits as one
+launch per round of passes, one hipGraph) against the CPU oracle, circuit by
+circuit: FP64 to 1e-12, FP32 to 2e-5 (probabilities)."""
 import numpy as np
 import pytest
 
@@ -33,17 +34,61 @@ def test_batch_matches_oracle(precision, tol, graph):
     b.close()
 
 
-@pytest.mark.parametrize('lmax,threads', [(0, 256), (5, 512), (7, 1024)])
-def test_batch_split_choices_agree(lmax, threads):
-    specs = workloads.batch(6, 5, n_range=(3, 8), depth_range=(10, 30))
+@pytest.mark.parametrize('lmax,threads,ipp', [(5, 256, 1), (6, 512, 64), (7, 1024, 1024)])
+def test_batch_split_choices_agree(lmax, threads, ipp):
+    specs = workloads.batch(8, 5, n_range=(3, 10), depth_range=(10, 30))
     ref = batch.from_workload(specs, 'fp64', lds_max_qubits=7, lds_threads=256)
     want, _, _ = ref.run(False)
     ref.close()
-    b = batch.from_workload(specs, 'fp64', lds_max_qubits=lmax, lds_threads=threads)
+    b = batch.from_workload(specs, 'fp64', lds_max_qubits=lmax, lds_threads=threads, items_per_pass=ipp)
     got, _, _ = b.run(True)
     b.close()
     for g, w in zip(got, want):
         assert np.max(np.abs(g - w)) <= 1e-13
+
+
+def b_circuits(specs):
+    return [(n, batch.fused_circuit_program(n, gates, decay_factor=1.0 - dr)) for n, gates, dr, _ in specs]
+
+
+def test_batching_workload_reduced_against_oracle():
+    """The batching workload's generator (widths 6-10 here, depths 20-60,
+    per-circuit decay and measurement rates) with every HBM-tier circuit in
+    the round kernels, graph replayed three times, FP64 and FP32."""
+    specs = workloads.batch(24, 3000, n_range=(6, 10), depth_range=(20, 60))
+    wants = [oracle_probs(n, gates, dr, pr) for n, gates, dr, pr in specs]
+    for precision, tol in (('fp64', 1e-12), ('fp32', 2e-5)):
+        b = batch.from_workload(specs, precision, lds_max_qubits=5, lds_threads=256, items_per_pass=64)
+        info = b.info()
+        assert info['hbm_circuits'] == 24
+        assert info['launches'] == max(len(c[1]) for c in b_circuits(specs)) + 2
+        for rep in range(3):
+            probs, us, ms = b.run(True)
+            for (n, _, _, _), got, want in zip(specs, probs, wants):
+                assert np.max(np.abs(got - want)) <= tol, (precision, n, rep)
+                assert abs(got.sum() - 1.0) <= (1e-12 if precision == 'fp64' else 1e-5)
+            assert np.all(us > 0) and ms > 0
+        b.close()
+
+
+def test_plain_programs_with_memory_noise_in_the_hbm_tier():
+    """Circuits built one op per gate plus a memory-noise op per level (the
+    HBM tier turns each into 6-digit tile passes) against the oracle."""
+    specs = workloads.batch(5, 41, n_range=(8, 11), depth_range=(10, 25))
+    circuits = [(n, batch.circuit_program(n, gates, decay_factor=1.0 - dr, decoherence_factor=0.99,
+                                          thermal_factor=0.8), 1.0 - pr) for n, gates, dr, pr in specs]
+    b = batch.Batch(circuits, 'fp64', lds_max_qubits=7, lds_threads=256, items_per_pass=16)
+    probs, _, _ = b.run(True)
+    b.close()
+    for (n, gates, dr, pr), got in zip(specs, probs):
+        qc = workloads.to_circuit(n, gates, barrier_after_each=True, measure=('ensemble', 'Z'))
+        from dmsim import assemble
+        exp = assemble(qc).experiments[0]
+        data, _, _ = refsim.run_experiment(exp.instructions, n, {
+            'decay_factor': 1.0 - dr, 'decoherence_factor': 0.99, 'thermal_factor': 0.8,
+            'depolarization_factor': 1.0 - pr, 'compute_densitymatrix': False})
+        want = np.array(list(data['ensemble_probability'].values()))
+        assert np.max(np.abs(got - want)) <= 1e-12, n
 
 
 @pytest.mark.parametrize('precision,tol', [('fp64', 1e-12), ('fp32', 2e-5)])
