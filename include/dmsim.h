@@ -188,6 +188,17 @@ typedef struct dm_op {
 
 int dm_run_program(dm_state *h, const dm_op *ops, int nops, const double *params, int64_t nparams);
 
+/* dm_run_program without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  The program image (tile
+ * operation records and block / fused-pass values, in the storage type) goes
+ * to a fixed 8 MiB device array that every call on the same GPU shares and
+ * that keeps the last call's contents; larger images are refused with
+ * DM_ERR_ARG before any work.  Calls that overlap on one GPU (other handles,
+ * other threads) race on that array.  Same results as dm_run_program (the
+ * same kernels). */
+int dm_run_program_noalloc(dm_state *h, const dm_op *ops, int nops, const double *params, int64_t nparams);
+
 /* Fusion cost model (no reference counterpart): a 3-layer perceptron
  * x -> relu(x W1 + b1) -> relu(. W2 + b2) -> . W3 + b3, FP32, whose forward,
  * backward (mean squared error) and Adam kernels run on the GPU.  Parameter

@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -2478,6 +2479,60 @@ int run_program(dm_state *h, const dm_op *ops, int nops, const double *params, i
   return launch_program<T>(h, pp);
 }
 
+}  // namespace
+
+// Program image of dm_run_program_noalloc: one fixed array per device, held
+// until the process ends.  In host code `scratch` names a host-side stand-in,
+// not device memory; its device address comes from hipGetSymbolAddress
+// (scratch_address), once per device.
+constexpr size_t kNoAllocScratchBytes = 8u << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// Device address of `scratch` on `device`, looked up on the first call for
+// that device with that device current (the caller's device is restored).
+int scratch_address(int device, void **out) {
+  if (device < 0 || device >= kMaxDevices) return fail(DM_ERR_ARG, "device %d out of range", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != device) HIP_TRY(hipSetDevice(device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return fail(DM_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return DM_OK;
+}
+
+// dm_run_program with the program image in the device's fixed scratch array
+// instead of the handle's growable buffer: stage on the host, refuse an
+// image larger than the array, copy it there after the handle's earlier work
+// has finished, enqueue the same kernels.
+template <typename T>
+int run_program_noalloc(dm_state *h, const dm_op *ops, int nops, const double *params, int64_t nparams) {
+  Prepared pp;
+  if (int rc = stage_program<T>(h, ops, nops, params, nparams, &pp)) return rc;
+  if (pp.image_bytes() > kNoAllocScratchBytes)
+    return fail(DM_ERR_ARG, "program image of %zu bytes exceeds the %zu-byte scratch of dm_run_program_noalloc",
+                pp.image_bytes(), kNoAllocScratchBytes);
+  if (!pp.tops.empty() || !pp.values.empty()) {
+    void *dev = nullptr;
+    if (int rc = scratch_address(h->device, &dev)) return rc;
+    if (int rc = upload_image(h, &pp, dev)) return rc;
+  }
+  return launch_program<T>(h, pp);
+}
+
 
 }  // namespace
 
@@ -3223,6 +3278,13 @@ int dm_run_program(dm_state *h, const dm_op *ops, int nops, const double *params
   if (nops < 0 || (nops > 0 && !ops)) return fail(DM_ERR_ARG, "bad op list");
   if (nparams < 0 || (nparams > 0 && !params)) return fail(DM_ERR_ARG, "bad parameter array");
   return DISPATCH(h, run_program, h, ops, nops, params, nparams);
+}
+
+int dm_run_program_noalloc(dm_state *h, const dm_op *ops, int nops, const double *params, int64_t nparams) {
+  if (int rc = check_handle(h)) return rc;
+  if (nops < 0 || (nops > 0 && !ops)) return fail(DM_ERR_ARG, "bad op list");
+  if (nparams < 0 || (nparams > 0 && !params)) return fail(DM_ERR_ARG, "bad parameter array");
+  return DISPATCH(h, run_program_noalloc, h, ops, nops, params, nparams);
 }
 
 int dm_batch_destroy(dm_batch *b) {

@@ -67,6 +67,8 @@ SIGNATURES = {
                                              c_int_p, c_int_p, c_float_p, ctypes.c_int, c_double_p]),
     'dm_run_program': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, c_double_p,
                                       ctypes.c_int64]),
+    'dm_run_program_noalloc': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, c_double_p,
+                                              ctypes.c_int64]),
     'dm_mlp_param_count': (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     'dm_mlp_init': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, c_float_p]),
     'dm_mlp_predict': (ctypes.c_int, [c_float_p, ctypes.c_int, ctypes.c_int, c_float_p, ctypes.c_int, c_float_p]),
