@@ -13,6 +13,7 @@
 #include <stdarg.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -755,7 +756,23 @@ __device__ __forceinline__ void tile_2q(T (&v)[4][4], const TileOp &op, const T 
 // stored at lds_swz of each tile index, and strided operations enumerate
 // their element groups in the digit order of TileOp::lanes, so that the 64
 // lanes of a wave fall on distinct banks.
-template <typename T, int M, bool DENSE, bool GEN2Q, int NT, bool SWZ>
+// Workgroup barrier between tile operations.  LDSB: wait only for this
+// wave's LDS accesses (lgkmcnt) before the barrier, so global->LDS loads of
+// a prefetched tile stay in flight across it; the empty asm statements keep
+// the compiler from moving memory accesses across the barrier.
+template <bool LDSB>
+__device__ __forceinline__ void tile_sync() {
+  if (LDSB) {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // vmcnt(63) expcnt(7) lgkmcnt(0)
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+  } else {
+    __syncthreads();
+  }
+}
+
+template <typename T, int M, bool DENSE, bool GEN2Q, int NT, bool SWZ, bool LDSB = false>
 __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, int nops, const T *__restrict__ prm) {
   constexpr int E = 1 << (2 * M);
   constexpr int G1 = (E / 4 + NT - 1) / NT;   // 1q groups per thread
@@ -904,7 +921,7 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
     } else if (DENSE && op.kind == TOP_DENSE) {
       if (M == 6) tile_dense<T, M>(s, op, pv);
     }
-    __syncthreads();
+    tile_sync<LDSB>();
   }
 }
 
@@ -1011,6 +1028,97 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
   for (int i = 0; i < QPT; ++i) {
     const int c = tid + i * NT;
     if (c < E / 4) stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), lds_get_quad(s, lds_swz<SWZ>(4 * c)));
+  }
+}
+
+// Global -> LDS copy of one tile with gfx950's asynchronous loads: lane l
+// of wave v, in step i, writes LDS quad i*NT + 64v + l (one 1 KB run per wave
+// and step) with the tile quad stored there (the swizzle is an involution).
+// 16-byte quads: FP32 only.  Completion is counted by vmcnt.
+template <int M, int NT, bool SWZ>
+__device__ __forceinline__ void tile_dma(float *lds, const float *a, uint64_t base, int w, int lomask,
+                                         const TileGeom &geo) {
+  constexpr int E = 1 << (2 * M);
+  constexpr int QPT = E / 4 / NT;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+#pragma unroll
+  for (int i = 0; i < QPT; ++i) {
+    const int c = tid + i * NT;
+    const float *src = a + base + tile_offset<float, M>(lds_swz<SWZ>(4 * c), w, lomask, geo);
+    float *dst = lds + 4 * (i * NT + wave * 64);
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+  }
+}
+
+template <int M, int NT, bool SWZ>
+__device__ __forceinline__ void tile_store(const float *lds, float *a, uint64_t base, int w, int lomask,
+                                           const TileGeom &geo) {
+  constexpr int E = 1 << (2 * M);
+  constexpr int QPT = E / 4 / NT;
+#pragma unroll
+  for (int i = 0; i < QPT; ++i) {
+    const int c = threadIdx.x + i * NT;
+    stp(a + base + tile_offset<float, M>(4 * c, w, lomask, geo), lds_get_quad(lds, lds_swz<SWZ>(4 * c)));
+  }
+}
+
+// Every outstanding global access of this wave (the prefetched tile and the
+// stores of the last one) is complete, then the workgroup meets.
+__device__ __forceinline__ void vm_sync() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// A fused pass over FP32 tiles with two LDS buffers per workgroup (one
+// workgroup per CU for 7-digit tiles: 2 x 64 KB): while the operations run
+// on the tile in one buffer, the next tile streams into the other with
+// asynchronous global->LDS loads, so a workgroup overlaps its own HBM reads
+// with its LDS work.  Workgroup b takes tiles b, b + grid, ...; the loop is
+// unrolled over the two buffers (two distinct LDS arrays, so the compiler
+// can tell the prefetch target from the buffer being read).
+template <int M, bool GEN2Q, int NT, bool SWZ>
+__global__ __launch_bounds__(NT) void k_tile_dma(float *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
+                                                 int nops, const float *__restrict__ prm) {
+  constexpr int E = 1 << (2 * M);
+  __shared__ __align__(16) float bufA[E];
+  __shared__ __align__(16) float bufB[E];
+  const int w = geo.w;
+  const int lomask = (1 << (2 * w)) - 1;
+  auto tile_base = [&](uint64_t tile) {
+    uint64_t base = 0, r = tile;
+    for (int j = 0; j < geo.nr; ++j) {
+      base |= (r & 3ull) << (2 * geo.rpos[j]);
+      r >>= 2;
+    }
+    return base;
+  };
+  uint64_t t = blockIdx.x;
+  if (t >= geo.tiles) return;  // uniform over the workgroup
+  uint64_t base = tile_base(t);
+  tile_dma<M, NT, SWZ>(bufA, a, base, w, lomask, geo);
+  vm_sync();
+  for (;;) {
+    uint64_t tn = t + gridDim.x;
+    uint64_t bn = tn < geo.tiles ? tile_base(tn) : 0;
+    if (tn < geo.tiles) tile_dma<M, NT, SWZ>(bufB, a, bn, w, lomask, geo);
+    tile_ops<float, M, false, GEN2Q, NT, SWZ, true>(bufA, ops, nops, prm);
+    tile_store<M, NT, SWZ>(bufA, a, base, w, lomask, geo);
+    vm_sync();
+    if (tn >= geo.tiles) break;
+    t = tn;
+    base = bn;
+    tn = t + gridDim.x;
+    bn = tn < geo.tiles ? tile_base(tn) : 0;
+    if (tn < geo.tiles) tile_dma<M, NT, SWZ>(bufA, a, bn, w, lomask, geo);
+    tile_ops<float, M, false, GEN2Q, NT, SWZ, true>(bufB, ops, nops, prm);
+    tile_store<M, NT, SWZ>(bufB, a, base, w, lomask, geo);
+    vm_sync();
+    if (tn >= geo.tiles) break;
+    t = tn;
+    base = bn;
   }
 }
 
@@ -1505,6 +1613,14 @@ struct ProfPending {
   int fam;
 };
 
+// A fused pass parsed from its parameter record (see DM_OP_FUSED in dmsim.h).
+struct FusedPass {
+  int m = 0;
+  TileGeom geo;
+  std::vector<TileOp> ops;     // param offsets relative to this pass's values
+  std::vector<double> vals;    // values in kernel layout (FP64; converted at upload)
+};
+
 struct dm_state {
   int n = 0;
   int dtype = DM_FP64;
@@ -1521,6 +1637,13 @@ struct dm_state {
   size_t mat_cap = 0;           // bytes
   double *d_aux = nullptr;      // gathered vectors / overlap operand
   size_t aux_cap = 0;           // bytes
+  // memory noise with a thermal feed as tile passes: the passes, and their
+  // records and values on the device, for the last (f, p, g)
+  std::vector<FusedPass> mn_passes;
+  std::vector<size_t> mn_top_off, mn_val_off;
+  void *d_mn = nullptr;
+  size_t mn_cap = 0;
+  double mn_key[3] = {0.0, 0.0, -1.0};
   // profiling
   bool prof = false;
   int prof_depth = 0;
@@ -1735,10 +1858,14 @@ int launch_tile_maps(dm_state *h, const int *pos, const double (*A)[4][4]) {
 }
 
 template <typename T>
+int launch_memory_noise_tiles(dm_state *h, double f, double p, double g);
+
+template <typename T>
 int launch_memory_noise(dm_state *h, double f, double p, double g) {
   const double od = sqrt(g) * f;
   const double dd = (1 - g) * (2 * p - 1);
   ProfScope ps(h, 3);
+  if (dd != 0.0 && h->n >= 6) return launch_memory_noise_tiles<T>(h, f, p, g);
   if (dd == 0.0) {
     constexpr int W = 16 / sizeof(T);
     const uint64_t units = h->N / W;
@@ -2100,13 +2227,6 @@ int validate_op(const dm_state *h, const dm_op &op, int i, int64_t nparams) {
   return DM_OK;
 }
 
-// A fused pass parsed from its parameter record (see DM_OP_FUSED in dmsim.h).
-struct FusedPass {
-  int m = 0;
-  TileGeom geo;
-  std::vector<TileOp> ops;     // param offsets relative to this pass's values
-  std::vector<double> vals;    // values in kernel layout (FP64; converted at upload)
-};
 
 int parse_fused(const dm_state *h, const double *p, int64_t avail, int opi, FusedPass *fp) {
   int64_t at = 0;
@@ -2278,6 +2398,18 @@ int stream_resident_per_cu() {
   return cached;
 }
 
+// Threads per workgroup of the double-buffered 7-digit FP32 tile kernel, 0
+// for the one-tile-per-workgroup kernels (environment DMSIM_TILE_DMA).
+int tile_dma_threads() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = getenv("DMSIM_TILE_DMA");
+    const int x = e ? atoi(e) : 0;
+    v = (x == 512 || x == 1024) ? x : 0;
+  }
+  return v;
+}
+
 template <typename T>
 int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T *d_vals) {
   ProfScope ps(h, 9);
@@ -2328,6 +2460,32 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
     if (fp.geo.ident) TILE_LAUNCH_L(MM, D, G, NTH, false); \
     else TILE_LAUNCH_L(MM, D, G, NTH, true);     \
   } while (0)
+  if constexpr (std::is_same<T, float>::value) {
+    // 7-digit FP32 passes on the double-buffered kernel (DMSIM_TILE_DMA = 512
+    // or 1024 threads per workgroup; one workgroup per CU)
+    const int dma = tile_dma_threads();
+    if (fp.m == 7 && !dense && dma) {
+      const unsigned g = (unsigned)std::min<uint64_t>(fp.geo.tiles, (uint64_t)h->cu_count);
+#define DMA_LAUNCH(G, NTH)                                                                                       \
+  do {                                                                                                           \
+    if (fp.geo.ident)                                                                                            \
+      hipLaunchKernelGGL((k_tile_dma<7, G, NTH, false>), dim3(g), dim3(NTH), 0, h->stream, a, fp.geo, d_ops, nops, \
+                         d_vals);                                                                                \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_tile_dma<7, G, NTH, true>), dim3(g), dim3(NTH), 0, h->stream, a, fp.geo, d_ops, nops,  \
+                         d_vals);                                                                                \
+  } while (0)
+      if (dma == 1024) {
+        if (gen) DMA_LAUNCH(true, 1024);
+        else DMA_LAUNCH(false, 1024);
+      } else {
+        if (gen) DMA_LAUNCH(true, 512);
+        else DMA_LAUNCH(false, 512);
+      }
+#undef DMA_LAUNCH
+      return last_launch("k_tile_dma");
+    }
+  }
   if (fp.m == 7 && gen) TILE_LAUNCH(7, false, true, 512);
   else if (fp.m == 7) TILE_LAUNCH(7, false, false, 512);
   else if (fp.m == 6 && dense) TILE_LAUNCH_L(6, true, true, 256, false);
@@ -2338,6 +2496,73 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
 #undef TILE_LAUNCH_L
 #undef TILE_LAUNCH
   return last_launch("k_tile_prog");
+}
+
+// Memory noise with a thermal feed, d = (1-g)(2p-1) != 0 (the z component
+// of every qubit takes d a_I, so the channel is not diagonal), as tile
+// passes of the 1-qubit channel R = [X: od x; Y: od y; Z: g z + d a_I] on
+// every position: the first pass holds positions 0 .. m-1 (runs of 4^m
+// coefficients), each further pass the three lowest positions (runs of 4^3)
+// and m - 3 new ones; m = 6 in FP64 (32 KB of LDS), 7 in FP32 (64 KB).
+// 1 + ceil((n - m) / (m - 3)) passes, against 1 + ceil((n - 1) / 3) for
+// 3-qubit register tiles.  The passes and their device records are kept per
+// handle for the last (f, p, g).
+template <typename T>
+int launch_memory_noise_tiles(dm_state *h, double f, double p, double g) {
+  const int n = h->n;
+  const int m = sizeof(T) == 4 && n >= 7 ? 7 : 6;
+  if (!(h->mn_key[0] == f && h->mn_key[1] == p && h->mn_key[2] == g)) {
+    const double od = sqrt(g) * f, dd = (1 - g) * (2 * p - 1);
+    const double R[12] = {0, od, 0, 0, 0, 0, od, 0, dd, 0, 0, g};
+    std::vector<FusedPass> passes;
+    auto add_pass = [&](int lo, int hi) -> int {  // channel on positions [lo, hi); tile filled from position 0
+      std::vector<double> rec = {(double)m, (double)(hi - lo)};
+      for (int pos = lo; pos < hi; ++pos) rec.push_back(n - 1 - pos);
+      rec.push_back(hi - lo);
+      for (int pos = lo; pos < hi; ++pos) {
+        const double head[7] = {(double)TOP_1Q, 1.0, (double)(n - 1 - pos), 0, 0, 0, 12.0};
+        rec.insert(rec.end(), head, head + 7);
+        rec.insert(rec.end(), R, R + 12);
+      }
+      FusedPass fp;
+      if (int rc = parse_fused(h, rec.data(), (int64_t)rec.size(), 0, &fp)) return rc;
+      passes.push_back(std::move(fp));
+      return DM_OK;
+    };
+    if (int rc = add_pass(0, std::min(n, m))) return rc;
+    for (int lo = m; lo < n; lo += m - 3)
+      if (int rc = add_pass(lo, std::min(n, lo + m - 3))) return rc;
+    // device image: [records of every pass | values (T)], values 256-byte aligned
+    std::vector<TileOp> tops;
+    std::vector<T> vals;
+    std::vector<size_t> top_off, val_off;
+    for (const FusedPass &fp : passes) {
+      top_off.push_back(tops.size());
+      val_off.push_back(vals.size());
+      tops.insert(tops.end(), fp.ops.begin(), fp.ops.end());
+      for (double v : fp.vals) vals.push_back((T)v);
+    }
+    const size_t vals_at = (tops.size() * sizeof(TileOp) + 255) & ~(size_t)255;
+    const size_t bytes = vals_at + vals.size() * sizeof(T);
+    HIP_TRY(hipStreamSynchronize(h->stream));  // earlier passes may still read the old image
+    if (int rc = ensure(&h->d_mn, &h->mn_cap, bytes)) return rc;
+    HIP_TRY(hipMemcpy(h->d_mn, tops.data(), tops.size() * sizeof(TileOp), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(static_cast<char *>(h->d_mn) + vals_at, vals.data(), vals.size() * sizeof(T),
+                      hipMemcpyHostToDevice));
+    for (size_t &o : val_off) o = vals_at + o * sizeof(T);
+    h->mn_passes = std::move(passes);
+    h->mn_top_off = std::move(top_off);
+    h->mn_val_off = std::move(val_off);
+    h->mn_key[0] = f;
+    h->mn_key[1] = p;
+    h->mn_key[2] = g;
+  }
+  for (size_t i = 0; i < h->mn_passes.size(); ++i) {
+    const TileOp *d_ops = static_cast<const TileOp *>(h->d_mn) + h->mn_top_off[i];
+    const T *d_vals = reinterpret_cast<const T *>(static_cast<const char *>(h->d_mn) + h->mn_val_off[i]);
+    if (int rc = launch_fused<T>(h, h->mn_passes[i], d_ops, d_vals)) return rc;
+  }
+  return DM_OK;
 }
 
 // A program split into a host phase (stage: validate, parse the fused
@@ -2532,7 +2757,6 @@ int run_program_noalloc(dm_state *h, const dm_op *ops, int nops, const double *p
   }
   return launch_program<T>(h, pp);
 }
-
 
 }  // namespace
 
@@ -3013,6 +3237,7 @@ int dm_destroy(dm_state *h) {
   if (h->d_red) hipFree(h->d_red);
   if (h->d_mat) hipFree(h->d_mat);
   if (h->d_aux) hipFree(h->d_aux);
+  if (h->d_mn) hipFree(h->d_mn);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
   for (auto &pp : h->pending) {

@@ -138,14 +138,22 @@ class Batch:
             pass
 
 
-def from_workload(specs, precision='fp64', lds_max_qubits=7, **kw):
+def from_workload(specs, precision='fp64', lds_max_qubits=7, programs=None, **kw):
     """Batch from dmsim.workloads.batch specs (n, gates, decay_rate, depol_rate):
     decay_factor = 1 - decay_rate, measurement retention = 1 - depol_rate.
-    Circuits above the LDS cutoff run as fused tile passes."""
+    Circuits above the LDS cutoff run as fused tile passes.  ``programs``, a
+    dict, keeps the built programs per (circuit, form) for later calls on the
+    same specs."""
     circuits = []
-    for n, gates, dr, pr in specs:
-        build = fused_circuit_program if n > lds_max_qubits else circuit_program
-        circuits.append((n, build(n, gates, decay_factor=1.0 - dr), 1.0 - pr))
+    for i, (n, gates, dr, pr) in enumerate(specs):
+        fused = n > lds_max_qubits
+        key = (i, fused)
+        prog = programs.get(key) if programs is not None else None
+        if prog is None:
+            prog = (fused_circuit_program if fused else circuit_program)(n, gates, decay_factor=1.0 - dr)
+            if programs is not None:
+                programs[key] = prog
+        circuits.append((n, prog, 1.0 - pr))
     return Batch(circuits, precision, lds_max_qubits=lds_max_qubits, **kw)
 
 
@@ -154,7 +162,7 @@ def _tune_key(specs, precision):
     return '%s:%s:%d:%d-%d' % (precision, native.device_info(0)['cu_count'], len(specs), min(ns), max(ns))
 
 
-def autotune(specs, precision='fp64', reps=2, use_cache=True):
+def autotune(specs, precision='fp64', reps=2, use_cache=True, programs=None):
     """Best (lds_max_qubits, lds_threads, items_per_pass) for this batch by
     measured time: the LDS cutoff and workgroup size first (items per pass
     256), then the items per pass at that cutoff."""
@@ -174,8 +182,11 @@ def autotune(specs, precision='fp64', reps=2, use_cache=True):
                 pass
     trials = {}
 
+    programs = {} if programs is None else programs
+
     def trial(lmax, thr, ipp):
-        b = from_workload(specs, precision, lds_max_qubits=lmax, lds_threads=thr, items_per_pass=ipp)
+        b = from_workload(specs, precision, lds_max_qubits=lmax, lds_threads=thr, items_per_pass=ipp,
+                          programs=programs)
         b.run(True)
         t = min(b.run(True)[2] for _ in range(reps))
         b.close()

@@ -114,7 +114,7 @@ def test_tsp_cx_dense(prec):
 
 @pytest.mark.parametrize('prec', PRECS)
 @pytest.mark.parametrize('fpg', [(0.9, 0.5, 0.99), (0.95, 1.0, 0.97), (0.8, 0.0, 0.9)])
-@pytest.mark.parametrize('n', [1, 2, 4, 7])
+@pytest.mark.parametrize('n', [1, 2, 4, 6, 7, 9, 11])
 def test_memory_noise(prec, fpg, n):
     rng = np.random.default_rng(300 + n)
     a = rand_state(n, rng)
@@ -123,6 +123,23 @@ def test_memory_noise(prec, fpg, n):
     f, p, g = fpg
     want = refsim.apply_memory_noise(a.reshape(n * [4]).copy(), n, f, p, g).reshape(-1)
     assert np.max(np.abs(st.download() - want)) <= tol(prec, n) * 4
+    st.close()
+
+
+@pytest.mark.parametrize('prec', PRECS)
+def test_memory_noise_repeated_with_changing_parameters(prec):
+    """The thermal-feed channel's tile passes are kept per handle for the
+    last parameters: the same parameters twice, then others, then the first
+    again, against the oracle applied in the same order."""
+    n = 10
+    rng = np.random.default_rng(77)
+    a = rand_state(n, rng)
+    st = make(n, prec, a)
+    want = a.reshape(n * [4]).copy()
+    for fpg in ((0.95, 0.8, 0.97), (0.95, 0.8, 0.97), (0.9, 0.3, 0.99), (0.95, 0.8, 0.97)):
+        st.memory_noise(*fpg)
+        want = refsim.apply_memory_noise(want, n, *fpg)
+    assert np.max(np.abs(st.download() - want.reshape(-1))) <= tol(prec, n) * 8
     st.close()
 
 
