@@ -13,7 +13,6 @@
 #include <stdarg.h>
 #include <stdint.h>
 #include <stdio.h>
-#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -615,6 +614,10 @@ struct TileOp {
   // the tile digits the operation leaves free (3 bits per digit, first digit
   // in bits 0-2); the first ones cover distinct bank fields (lds_swz).
   int32_t lanes;
+  // FP32 operations on digits >= 1 with 8- or 16-byte LDS accesses along
+  // digit 0: the order of the free digits other than digit 0, the first two
+  // covering the bank fields of LDS index bits 2-3 and 4-5.
+  int32_t qlanes;
 };
 
 struct TileGeom {
@@ -756,23 +759,11 @@ __device__ __forceinline__ void tile_2q(T (&v)[4][4], const TileOp &op, const T 
 // stored at lds_swz of each tile index, and strided operations enumerate
 // their element groups in the digit order of TileOp::lanes, so that the 64
 // lanes of a wave fall on distinct banks.
-// Workgroup barrier between tile operations.  LDSB: wait only for this
-// wave's LDS accesses (lgkmcnt) before the barrier, so global->LDS loads of
-// a prefetched tile stay in flight across it; the empty asm statements keep
-// the compiler from moving memory accesses across the barrier.
-template <bool LDSB>
-__device__ __forceinline__ void tile_sync() {
-  if (LDSB) {
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_waitcnt(0xC07F);  // vmcnt(63) expcnt(7) lgkmcnt(0)
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-  } else {
-    __syncthreads();
-  }
-}
-
-template <typename T, int M, bool DENSE, bool GEN2Q, int NT, bool SWZ, bool LDSB = false>
+// VEC: FP32 operations on digits >= 1 take 16-byte (single-qubit) or 8-byte
+// (two-qubit) LDS accesses along digit 0; they need more registers, so the
+// 64-register one-tile kernel of 6-digit tiles leaves them off (measured on
+// MI355X: its scratch spills cost more than the accesses save).
+template <typename T, int M, bool DENSE, bool GEN2Q, int NT, bool SWZ, bool VEC = true>
 __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, int nops, const T *__restrict__ prm) {
   constexpr int E = 1 << (2 * M);
   constexpr int G1 = (E / 4 + NT - 1) / NT;   // 1q groups per thread
@@ -790,6 +781,34 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
       for (int x = 0; x < 3; ++x)
 #pragma unroll
         for (int y = 0; y < 4; ++y) r[x][y] = pv[4 * x + y];
+      if constexpr (VEC && sizeof(T) == 4) {
+        if (j >= 1) {
+          // FP32, digit >= 1: a thread takes the four groups that differ in
+          // digit 0 (one 16-byte LDS access per component: 7 accesses for
+          // 16 coefficients instead of 28)
+          constexpr int SG = E / 16;
+          const int st = 1 << (2 * j);
+          int ko[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) ko[c] = lds_swz<SWZ>(c * st);
+          for (int sg = tid; sg < SG; sg += NT) {
+            const int b = lds_swz<SWZ>(lane_deposit(sg, op.qlanes, M - 2));
+            Q x[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) x[c] = *reinterpret_cast<const Q *>(&s[b ^ ko[c]]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              Q y;
+#pragma unroll
+              for (int e = 0; e < 4; ++e)
+                y.v[e] = r[c][0] * x[0].v[e] + r[c][1] * x[1].v[e] + r[c][2] * x[2].v[e] + r[c][3] * x[3].v[e];
+              *reinterpret_cast<Q *>(&s[b ^ ko[c + 1]]) = y;
+            }
+          }
+          __syncthreads();
+          continue;
+        }
+      }
       // A thread's groups go in chunks of GC: all loads of a chunk are
       // issued before its stores (LDS latency overlaps), and the chunk
       // bounds the registers the op holds.
@@ -847,6 +866,49 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
     } else if (op.kind == TOP_CX || (GEN2Q && op.kind == TOP_2Q)) {
       const int jc = op.d[0], jt = op.d[1];
       const int jl = min(jc, jt), jh = max(jc, jt);
+      if constexpr (VEC && sizeof(T) == 4) {
+        if (jl >= 1) {
+          // FP32, both digits >= 1: a thread takes the two groups that differ
+          // in the low bit of digit 0 (one 8-byte LDS access per pair
+          // component); the high bit of digit 0 is the lowest lane bit
+          using P2 = Pack<T, 2>;
+          constexpr int UN = E / 32;
+          int kc[4], kt[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            kc[u] = lds_swz<SWZ>(u << (2 * jc));
+            kt[u] = lds_swz<SWZ>(u << (2 * jt));
+          }
+          for (int un = tid; un < UN; un += NT) {
+            const int b = lds_swz<SWZ>(((un & 1) << 1) | lane_deposit(un >> 1, op.qlanes, M - 3));
+            P2 x[4][4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+              for (int t = 0; t < 4; ++t) x[c][t] = *reinterpret_cast<const P2 *>(&s[b ^ kc[c] ^ kt[t]]);
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+              T v[4][4];
+#pragma unroll
+              for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) v[c][t] = x[c][t].v[e];
+              tile_2q<T, GEN2Q>(v, op, pv);
+#pragma unroll
+              for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) x[c][t].v[e] = v[c][t];
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+              for (int t = 0; t < 4; ++t)
+                if (c | t) *reinterpret_cast<P2 *>(&s[b ^ kc[c] ^ kt[t]]) = x[c][t];
+          }
+          __syncthreads();
+          continue;
+        }
+      }
       if (SWZ && jl == 0) {
         // one digit is tile digit 0: the four quads along the other digit
         // are one vector access each
@@ -921,7 +983,7 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
     } else if (DENSE && op.kind == TOP_DENSE) {
       if (M == 6) tile_dense<T, M>(s, op, pv);
     }
-    tile_sync<LDSB>();
+    __syncthreads();
   }
 }
 
@@ -1023,102 +1085,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
     if (c < E / 4) lds_put_quad(s, lds_swz<SWZ>(4 * c), ldp<Q>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo)));
   }
   __syncthreads();
-  tile_ops<T, M, false, GEN2Q, NT, SWZ>(s, ops, nops, prm);
+  tile_ops<T, M, false, GEN2Q, NT, SWZ, (M >= 7)>(s, ops, nops, prm);
 #pragma unroll
   for (int i = 0; i < QPT; ++i) {
     const int c = tid + i * NT;
     if (c < E / 4) stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), lds_get_quad(s, lds_swz<SWZ>(4 * c)));
-  }
-}
-
-// Global -> LDS copy of one tile with gfx950's asynchronous loads: lane l
-// of wave v, in step i, writes LDS quad i*NT + 64v + l (one 1 KB run per wave
-// and step) with the tile quad stored there (the swizzle is an involution).
-// 16-byte quads: FP32 only.  Completion is counted by vmcnt.
-template <int M, int NT, bool SWZ>
-__device__ __forceinline__ void tile_dma(float *lds, const float *a, uint64_t base, int w, int lomask,
-                                         const TileGeom &geo) {
-  constexpr int E = 1 << (2 * M);
-  constexpr int QPT = E / 4 / NT;
-  const int tid = threadIdx.x;
-  const int wave = tid >> 6;
-#pragma unroll
-  for (int i = 0; i < QPT; ++i) {
-    const int c = tid + i * NT;
-    const float *src = a + base + tile_offset<float, M>(lds_swz<SWZ>(4 * c), w, lomask, geo);
-    float *dst = lds + 4 * (i * NT + wave * 64);
-    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
-  }
-}
-
-template <int M, int NT, bool SWZ>
-__device__ __forceinline__ void tile_store(const float *lds, float *a, uint64_t base, int w, int lomask,
-                                           const TileGeom &geo) {
-  constexpr int E = 1 << (2 * M);
-  constexpr int QPT = E / 4 / NT;
-#pragma unroll
-  for (int i = 0; i < QPT; ++i) {
-    const int c = threadIdx.x + i * NT;
-    stp(a + base + tile_offset<float, M>(4 * c, w, lomask, geo), lds_get_quad(lds, lds_swz<SWZ>(4 * c)));
-  }
-}
-
-// Every outstanding global access of this wave (the prefetched tile and the
-// stores of the last one) is complete, then the workgroup meets.
-__device__ __forceinline__ void vm_sync() {
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_waitcnt(0);
-  __builtin_amdgcn_s_barrier();
-  asm volatile("" ::: "memory");
-}
-
-// A fused pass over FP32 tiles with two LDS buffers per workgroup (one
-// workgroup per CU for 7-digit tiles: 2 x 64 KB): while the operations run
-// on the tile in one buffer, the next tile streams into the other with
-// asynchronous global->LDS loads, so a workgroup overlaps its own HBM reads
-// with its LDS work.  Workgroup b takes tiles b, b + grid, ...; the loop is
-// unrolled over the two buffers (two distinct LDS arrays, so the compiler
-// can tell the prefetch target from the buffer being read).
-template <int M, bool GEN2Q, int NT, bool SWZ>
-__global__ __launch_bounds__(NT) void k_tile_dma(float *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
-                                                 int nops, const float *__restrict__ prm) {
-  constexpr int E = 1 << (2 * M);
-  __shared__ __align__(16) float bufA[E];
-  __shared__ __align__(16) float bufB[E];
-  const int w = geo.w;
-  const int lomask = (1 << (2 * w)) - 1;
-  auto tile_base = [&](uint64_t tile) {
-    uint64_t base = 0, r = tile;
-    for (int j = 0; j < geo.nr; ++j) {
-      base |= (r & 3ull) << (2 * geo.rpos[j]);
-      r >>= 2;
-    }
-    return base;
-  };
-  uint64_t t = blockIdx.x;
-  if (t >= geo.tiles) return;  // uniform over the workgroup
-  uint64_t base = tile_base(t);
-  tile_dma<M, NT, SWZ>(bufA, a, base, w, lomask, geo);
-  vm_sync();
-  for (;;) {
-    uint64_t tn = t + gridDim.x;
-    uint64_t bn = tn < geo.tiles ? tile_base(tn) : 0;
-    if (tn < geo.tiles) tile_dma<M, NT, SWZ>(bufB, a, bn, w, lomask, geo);
-    tile_ops<float, M, false, GEN2Q, NT, SWZ, true>(bufA, ops, nops, prm);
-    tile_store<M, NT, SWZ>(bufA, a, base, w, lomask, geo);
-    vm_sync();
-    if (tn >= geo.tiles) break;
-    t = tn;
-    base = bn;
-    tn = t + gridDim.x;
-    bn = tn < geo.tiles ? tile_base(tn) : 0;
-    if (tn < geo.tiles) tile_dma<M, NT, SWZ>(bufA, a, bn, w, lomask, geo);
-    tile_ops<float, M, false, GEN2Q, NT, SWZ, true>(bufB, ops, nops, prm);
-    tile_store<M, NT, SWZ>(bufB, a, base, w, lomask, geo);
-    vm_sync();
-    if (tn >= geo.tiles) break;
-    t = tn;
-    base = bn;
   }
 }
 
@@ -2366,6 +2337,28 @@ int parse_fused(const dm_state *h, const double *p, int64_t avail, int opi, Fuse
       for (int k = 0; k < nf; ++k) packed |= ord[k] << (3 * k);
       op.lanes = packed;
     }
+  for (TileOp &op : fp->ops) {
+    if (op.kind == TOP_DENSE) continue;
+    const int nd = op.kind == TOP_1Q ? 1 : 2;
+    bool busy[8] = {false};
+    busy[0] = true;
+    for (int x = 0; x < nd; ++x) busy[op.d[x]] = true;
+    int ord[8], nf = 0;
+    auto take = [&](int d) {
+      if (d < m && !busy[d]) {
+        busy[d] = true;
+        ord[nf++] = d;
+        return true;
+      }
+      return false;
+    };
+    if (!take(1)) take(3);
+    if (!take(2)) take(4);
+    for (int d = 1; d < m; ++d) take(d);
+    int packed = 0;
+    for (int k = 0; k < nf; ++k) packed |= ord[k] << (3 * k);
+    op.qlanes = packed;
+  }
   return DM_OK;
 }
 
@@ -2396,18 +2389,6 @@ int stream_resident_per_cu() {
     cached = nb;
   }
   return cached;
-}
-
-// Threads per workgroup of the double-buffered 7-digit FP32 tile kernel, 0
-// for the one-tile-per-workgroup kernels (environment DMSIM_TILE_DMA).
-int tile_dma_threads() {
-  static int v = -1;
-  if (v < 0) {
-    const char *e = getenv("DMSIM_TILE_DMA");
-    const int x = e ? atoi(e) : 0;
-    v = (x == 512 || x == 1024) ? x : 0;
-  }
-  return v;
 }
 
 template <typename T>
@@ -2460,32 +2441,6 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
     if (fp.geo.ident) TILE_LAUNCH_L(MM, D, G, NTH, false); \
     else TILE_LAUNCH_L(MM, D, G, NTH, true);     \
   } while (0)
-  if constexpr (std::is_same<T, float>::value) {
-    // 7-digit FP32 passes on the double-buffered kernel (DMSIM_TILE_DMA = 512
-    // or 1024 threads per workgroup; one workgroup per CU)
-    const int dma = tile_dma_threads();
-    if (fp.m == 7 && !dense && dma) {
-      const unsigned g = (unsigned)std::min<uint64_t>(fp.geo.tiles, (uint64_t)h->cu_count);
-#define DMA_LAUNCH(G, NTH)                                                                                       \
-  do {                                                                                                           \
-    if (fp.geo.ident)                                                                                            \
-      hipLaunchKernelGGL((k_tile_dma<7, G, NTH, false>), dim3(g), dim3(NTH), 0, h->stream, a, fp.geo, d_ops, nops, \
-                         d_vals);                                                                                \
-    else                                                                                                         \
-      hipLaunchKernelGGL((k_tile_dma<7, G, NTH, true>), dim3(g), dim3(NTH), 0, h->stream, a, fp.geo, d_ops, nops,  \
-                         d_vals);                                                                                \
-  } while (0)
-      if (dma == 1024) {
-        if (gen) DMA_LAUNCH(true, 1024);
-        else DMA_LAUNCH(false, 1024);
-      } else {
-        if (gen) DMA_LAUNCH(true, 512);
-        else DMA_LAUNCH(false, 512);
-      }
-#undef DMA_LAUNCH
-      return last_launch("k_tile_dma");
-    }
-  }
   if (fp.m == 7 && gen) TILE_LAUNCH(7, false, true, 512);
   else if (fp.m == 7) TILE_LAUNCH(7, false, false, 512);
   else if (fp.m == 6 && dense) TILE_LAUNCH_L(6, true, true, 256, false);
