@@ -781,28 +781,32 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
       for (int x = 0; x < 3; ++x)
 #pragma unroll
         for (int y = 0; y < 4; ++y) r[x][y] = pv[4 * x + y];
-      if constexpr (VEC && sizeof(T) == 4) {
+      if constexpr (VEC) {
         if (j >= 1) {
-          // FP32, digit >= 1: a thread takes the four groups that differ in
-          // digit 0 (one 16-byte LDS access per component: 7 accesses for
-          // 16 coefficients instead of 28)
-          constexpr int SG = E / 16;
+          // digit >= 1: a thread takes the groups that differ in digit 0 and
+          // share one 16-byte LDS access per component: all four in FP32
+          // (7 accesses for 16 coefficients instead of 28), two in FP64 (the
+          // high bit of digit 0 is then the lowest lane bit)
+          constexpr int VW = 16 / sizeof(T);
+          using VV = Pack<T, VW>;
+          constexpr int SG = E / 4 / VW;
           const int st = 1 << (2 * j);
           int ko[4];
 #pragma unroll
           for (int c = 0; c < 4; ++c) ko[c] = lds_swz<SWZ>(c * st);
           for (int sg = tid; sg < SG; sg += NT) {
-            const int b = lds_swz<SWZ>(lane_deposit(sg, op.qlanes, M - 2));
-            Q x[4];
+            const int lo = VW == 2 ? ((sg & 1) << 1) : 0;
+            const int b = lds_swz<SWZ>(lo | lane_deposit(VW == 2 ? (sg >> 1) : sg, op.qlanes, M - 2));
+            VV x[4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) x[c] = *reinterpret_cast<const Q *>(&s[b ^ ko[c]]);
+            for (int c = 0; c < 4; ++c) x[c] = *reinterpret_cast<const VV *>(&s[b ^ ko[c]]);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-              Q y;
+              VV y;
 #pragma unroll
-              for (int e = 0; e < 4; ++e)
+              for (int e = 0; e < VW; ++e)
                 y.v[e] = r[c][0] * x[0].v[e] + r[c][1] * x[1].v[e] + r[c][2] * x[2].v[e] + r[c][3] * x[3].v[e];
-              *reinterpret_cast<Q *>(&s[b ^ ko[c + 1]]) = y;
+              *reinterpret_cast<VV *>(&s[b ^ ko[c + 1]]) = y;
             }
           }
           __syncthreads();
