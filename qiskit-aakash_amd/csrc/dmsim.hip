@@ -604,9 +604,15 @@ __global__ __launch_bounds__(kBlock) void k_block_mfma(T *__restrict__ a, const 
 //   TOP_DENSE  k = 3 or 4 digits d0..d{k-1}: 4^k x 4^k PTM on the matrix
 //              cores, values stored transposed (ST[col][row]); the other
 //              tile digits index the 4^(M-k) columns.
+//   TOP_STAGE  digits d0 < d1 and k members, the k records that follow it
+//              (TOP_1Q on d0 or d1, TOP_CX or TOP_2Q on {d0, d1} in either
+//              order), applied in order to each group of 16 coefficients in
+//              registers: one LDS round trip for all of them.
+// The kernels run TOP_STAGE and TOP_DENSE records; the host turns every
+// other operation of a pass into a stage member (group_stages).
 // ---------------------------------------------------------------------------
 
-enum { TOP_1Q = 1, TOP_CX = 2, TOP_2Q = 3, TOP_DENSE = 4 };
+enum { TOP_1Q = 1, TOP_CX = 2, TOP_2Q = 3, TOP_DENSE = 4, TOP_STAGE = 6 };
 
 struct TileOp {
   int32_t kind, k, d[4], param;
@@ -614,9 +620,9 @@ struct TileOp {
   // the tile digits the operation leaves free (3 bits per digit, first digit
   // in bits 0-2); the first ones cover distinct bank fields (lds_swz).
   int32_t lanes;
-  // FP32 operations on digits >= 1 with 8- or 16-byte LDS accesses along
-  // digit 0: the order of the free digits other than digit 0, the first two
-  // covering the bank fields of LDS index bits 2-3 and 4-5.
+  // FP32 stages on digits >= 1 with 8-byte LDS accesses along digit 0: the
+  // order of the free digits other than digit 0, the first two covering the
+  // bank fields of LDS index bits 2-3 and 4-5.
   int32_t qlanes;
 };
 
@@ -650,6 +656,32 @@ __device__ __forceinline__ int lane_deposit(int g, int ord, int nf) {
   int b = 0;
   for (int k = 0; k < nf; ++k) b |= ((g >> (2 * k)) & 3) << (2 * ((ord >> (3 * k)) & 7));
   return b;
+}
+
+// A quad (4 consecutive coefficients: 16 bytes in FP32, 32 in FP64) from or
+// to HBM, with the nontemporal hint when NT.
+template <typename T, bool NT>
+__device__ __forceinline__ Pack<T, 4> ldq(const T *p) {
+  constexpr int H = 16 / sizeof(T);
+  Pack<T, 4> q;
+#pragma unroll
+  for (int h = 0; h < 4 / H; ++h) {
+    const Pack<T, H> v = ld16<T, NT>(p + h * H);
+#pragma unroll
+    for (int e = 0; e < H; ++e) q.v[h * H + e] = v.v[e];
+  }
+  return q;
+}
+template <typename T, bool NT>
+__device__ __forceinline__ void stq(T *p, const Pack<T, 4> &q) {
+  constexpr int H = 16 / sizeof(T);
+#pragma unroll
+  for (int h = 0; h < 4 / H; ++h) {
+    Pack<T, H> v;
+#pragma unroll
+    for (int e = 0; e < H; ++e) v.v[e] = q.v[h * H + e];
+    st16<T, NT>(p + h * H, v);
+  }
 }
 
 template <typename T>
@@ -729,263 +761,174 @@ __device__ __forceinline__ uint64_t tile_offset(int t, int w, int lomask, const 
   return off;
 }
 
-// A two-qubit operation on the 16 components v[c][t] (c: digit op.d[0],
-// t: digit op.d[1]) of one pair group, in place: the CX signed permutation
-// with its optional per-pair scale, or (GEN2Q) a 16x16 PTM whose first row
-// is the identity's.
-template <typename T, bool GEN2Q>
-__device__ __forceinline__ void tile_2q(T (&v)[4][4], const TileOp &op, const T *__restrict__ pv) {
-  if (!GEN2Q || op.kind == TOP_CX) {
-    Scale16<T> sv;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) sv.s[q] = op.k ? pv[q] : T(1);
-    cx_map<T, T, true, true>(v, sv);
-  } else {
-    T w[16];
-#pragma unroll
-    for (int r = 1; r < 16; ++r) {
-      T acc = T(0);
-#pragma unroll
-      for (int c = 0; c < 16; ++c) acc += pv[16 * r + c] * v[c >> 2][c & 3];
-      w[r] = acc;
-    }
-#pragma unroll
-    for (int r = 1; r < 16; ++r) v[r >> 2][r & 3] = w[r];
-  }
-}
-
-// The operations of a tile program on the LDS tile s (all threads of the
-// workgroup take part; a barrier follows each operation).  SWZ: the tile is
-// stored at lds_swz of each tile index, and strided operations enumerate
-// their element groups in the digit order of TileOp::lanes, so that the 64
-// lanes of a wave fall on distinct banks.
-// VEC: FP32 operations on digits >= 1 take 16-byte (single-qubit) or 8-byte
-// (two-qubit) LDS accesses along digit 0; they need more registers, so the
-// 64-register one-tile kernel of 6-digit tiles leaves them off (measured on
-// MI355X: its scratch spills cost more than the accesses save).
-template <typename T, int M, bool DENSE, bool GEN2Q, int NT, bool SWZ, bool VEC = true>
-__device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, int nops, const T *__restrict__ prm) {
-  constexpr int E = 1 << (2 * M);
-  constexpr int G1 = (E / 4 + NT - 1) / NT;   // 1q groups per thread
-  constexpr int GCMAX = sizeof(T) == 8 ? 2 : 4;
-  constexpr int GC = G1 < GCMAX ? G1 : GCMAX;  // 1q groups per chunk
-  using Q = Pack<T, 4>;
-  const int tid = threadIdx.x;
-  for (int i = 0; i < nops; ++i) {
-    const TileOp op = ops[i];
-    const T *pv = prm + op.param;
-    if (op.kind == TOP_1Q) {
-      const int j = op.d[0];
+// The member operations of a stage on NG groups of 16 coefficients in
+// registers: v[g][x][y], x the component index along stage digit a (the
+// lower one), y along the other digit.
+template <typename T, bool GEN2Q, int NG>
+__device__ __forceinline__ void stage_apply(T (&v)[NG][4][4], int a, const TileOp *__restrict__ mem, int nm,
+                                            const T *__restrict__ prm) {
+  for (int q = 0; q < nm; ++q) {
+    const TileOp m = mem[q];
+    const T *pv = prm + m.param;
+    if (m.kind == TOP_1Q) {
       T r[3][4];
 #pragma unroll
       for (int x = 0; x < 3; ++x)
 #pragma unroll
         for (int y = 0; y < 4; ++y) r[x][y] = pv[4 * x + y];
-      if constexpr (VEC) {
-        if (j >= 1) {
-          // digit >= 1: a thread takes the groups that differ in digit 0 and
-          // share one 16-byte LDS access per component: all four in FP32
-          // (7 accesses for 16 coefficients instead of 28), two in FP64 (the
-          // high bit of digit 0 is then the lowest lane bit)
-          constexpr int VW = 16 / sizeof(T);
-          using VV = Pack<T, VW>;
-          constexpr int SG = E / 4 / VW;
-          const int st = 1 << (2 * j);
-          int ko[4];
+      if (m.d[0] == a) {
 #pragma unroll
-          for (int c = 0; c < 4; ++c) ko[c] = lds_swz<SWZ>(c * st);
-          for (int sg = tid; sg < SG; sg += NT) {
-            const int lo = VW == 2 ? ((sg & 1) << 1) : 0;
-            const int b = lds_swz<SWZ>(lo | lane_deposit(VW == 2 ? (sg >> 1) : sg, op.qlanes, M - 2));
-            VV x[4];
+        for (int g = 0; g < NG; ++g)
 #pragma unroll
-            for (int c = 0; c < 4; ++c) x[c] = *reinterpret_cast<const VV *>(&s[b ^ ko[c]]);
+          for (int y = 0; y < 4; ++y) {
+            const T x0 = v[g][0][y], x1 = v[g][1][y], x2 = v[g][2][y], x3 = v[g][3][y];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              VV y;
-#pragma unroll
-              for (int e = 0; e < VW; ++e)
-                y.v[e] = r[c][0] * x[0].v[e] + r[c][1] * x[1].v[e] + r[c][2] * x[2].v[e] + r[c][3] * x[3].v[e];
-              *reinterpret_cast<VV *>(&s[b ^ ko[c + 1]]) = y;
-            }
+            for (int c = 0; c < 3; ++c) v[g][c + 1][y] = r[c][0] * x0 + r[c][1] * x1 + r[c][2] * x2 + r[c][3] * x3;
           }
-          __syncthreads();
-          continue;
-        }
-      }
-      // A thread's groups go in chunks of GC: all loads of a chunk are
-      // issued before its stores (LDS latency overlaps), and the chunk
-      // bounds the registers the op holds.
-      if (j == 0) {
-        // the four components of a group are adjacent: one vector LDS access
-#pragma unroll
-        for (int q0 = 0; q0 < G1; q0 += GC) {
-          Q x[GC];
-#pragma unroll
-          for (int q = 0; q < GC; ++q) {
-            const int g = tid + (q0 + q) * NT;
-            if (g < E / 4) x[q] = *reinterpret_cast<const Q *>(&s[lds_swz<SWZ>(4 * g)]);
-          }
-#pragma unroll
-          for (int q = 0; q < GC; ++q) {
-            const int g = tid + (q0 + q) * NT;
-            if (g >= E / 4) continue;
-            Q y;
-            y.v[0] = x[q].v[0];
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-              y.v[c + 1] = r[c][0] * x[q].v[0] + r[c][1] * x[q].v[1] + r[c][2] * x[q].v[2] + r[c][3] * x[q].v[3];
-            *reinterpret_cast<Q *>(&s[lds_swz<SWZ>(4 * g)]) = y;
-          }
-        }
       } else {
-        const int st = 1 << (2 * j);
-        int ko[4];  // LDS offsets of the four components (XOR masks when swizzled)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) ko[c] = lds_swz<SWZ>(c * st);
+        for (int g = 0; g < NG; ++g)
 #pragma unroll
-        for (int q0 = 0; q0 < G1; q0 += GC) {
-          T x[GC][4];
-          int bb[GC];
+          for (int x = 0; x < 4; ++x) {
+            const T x0 = v[g][x][0], x1 = v[g][x][1], x2 = v[g][x][2], x3 = v[g][x][3];
 #pragma unroll
-          for (int q = 0; q < GC; ++q) {
-            const int g = tid + (q0 + q) * NT;
-            bb[q] = SWZ ? lds_swz<true>(lane_deposit(g, op.lanes, M - 1)) : ins_zero(g, j);
-            if (g < E / 4) {
-#pragma unroll
-              for (int c = 0; c < 4; ++c) x[q][c] = s[SWZ ? (bb[q] ^ ko[c]) : (bb[q] + ko[c])];
-            }
+            for (int c = 0; c < 3; ++c) v[g][x][c + 1] = r[c][0] * x0 + r[c][1] * x1 + r[c][2] * x2 + r[c][3] * x3;
           }
-#pragma unroll
-          for (int q = 0; q < GC; ++q) {
-            const int g = tid + (q0 + q) * NT;
-            if (g >= E / 4) continue;
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-              s[SWZ ? (bb[q] ^ ko[c + 1]) : (bb[q] + ko[c + 1])] =
-                  r[c][0] * x[q][0] + r[c][1] * x[q][1] + r[c][2] * x[q][2] + r[c][3] * x[q][3];
-          }
-        }
       }
-    } else if (op.kind == TOP_CX || (GEN2Q && op.kind == TOP_2Q)) {
-      const int jc = op.d[0], jt = op.d[1];
-      const int jl = min(jc, jt), jh = max(jc, jt);
-      if constexpr (VEC && sizeof(T) == 4) {
-        if (jl >= 1) {
-          // FP32, both digits >= 1: a thread takes the two groups that differ
-          // in the low bit of digit 0 (one 8-byte LDS access per pair
-          // component); the high bit of digit 0 is the lowest lane bit
-          using P2 = Pack<T, 2>;
-          constexpr int UN = E / 32;
-          int kc[4], kt[4];
+    } else if (!GEN2Q || m.kind == TOP_CX) {
+      Scale16<T> sv;
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            kc[u] = lds_swz<SWZ>(u << (2 * jc));
-            kt[u] = lds_swz<SWZ>(u << (2 * jt));
-          }
-          for (int un = tid; un < UN; un += NT) {
-            const int b = lds_swz<SWZ>(((un & 1) << 1) | lane_deposit(un >> 1, op.qlanes, M - 3));
-            P2 x[4][4];
+      for (int u = 0; u < 16; ++u) sv.s[u] = m.k ? pv[u] : T(1);
+      // control digit a: pair (c, t) is v[c][t]; control digit b: v[t][c]
+      if (m.d[0] == a) {
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-              for (int t = 0; t < 4; ++t) x[c][t] = *reinterpret_cast<const P2 *>(&s[b ^ kc[c] ^ kt[t]]);
-#pragma unroll
-            for (int e = 0; e < 2; ++e) {
-              T v[4][4];
-#pragma unroll
-              for (int c = 0; c < 4; ++c)
-#pragma unroll
-                for (int t = 0; t < 4; ++t) v[c][t] = x[c][t].v[e];
-              tile_2q<T, GEN2Q>(v, op, pv);
-#pragma unroll
-              for (int c = 0; c < 4; ++c)
-#pragma unroll
-                for (int t = 0; t < 4; ++t) x[c][t].v[e] = v[c][t];
-            }
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-              for (int t = 0; t < 4; ++t)
-                if (c | t) *reinterpret_cast<P2 *>(&s[b ^ kc[c] ^ kt[t]]) = x[c][t];
-          }
-          __syncthreads();
-          continue;
-        }
-      }
-      if (SWZ && jl == 0) {
-        // one digit is tile digit 0: the four quads along the other digit
-        // are one vector access each
-        int ko[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) ko[u] = lds_swz<true>(u << (2 * jh));
-        for (int g = tid; g < E / 16; g += NT) {
-          const int b = lds_swz<true>(lane_deposit(g, op.lanes, M - 2));
-          Q qd[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) qd[u] = *reinterpret_cast<const Q *>(&s[b ^ ko[u]]);
-          T v[4][4];
-          if (jc == 0) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-              for (int t = 0; t < 4; ++t) v[c][t] = qd[t].v[c];
-          } else {
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-              for (int t = 0; t < 4; ++t) v[c][t] = qd[c].v[t];
-          }
-          tile_2q<T, GEN2Q>(v, op, pv);
-          if (jc == 0) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-              for (int t = 0; t < 4; ++t) qd[t].v[c] = v[c][t];
-          } else {
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-              for (int t = 0; t < 4; ++t) qd[c].v[t] = v[c][t];
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u) *reinterpret_cast<Q *>(&s[b ^ ko[u]]) = qd[u];
-        }
+        for (int g = 0; g < NG; ++g) cx_map<T, T, true, true>(v[g], sv);
       } else {
-        int kc[4], kt[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          kc[u] = lds_swz<SWZ>(u << (2 * jc));
-          kt[u] = lds_swz<SWZ>(u << (2 * jt));
+        for (int g = 0; g < NG; ++g) cx_map<T, T, false, true>(v[g], sv);
+      }
+    } else {
+      // 16x16 PTM, pair index 4 i_d0 + i_d1 (first row the identity's)
+      const bool lo_first = m.d[0] == a;
+#pragma unroll
+      for (int g = 0; g < NG; ++g) {
+        T w[16];
+#pragma unroll
+        for (int r = 1; r < 16; ++r) {
+          T acc = T(0);
+#pragma unroll
+          for (int c = 0; c < 16; ++c) acc += pv[16 * r + c] * (lo_first ? v[g][c >> 2][c & 3] : v[g][c & 3][c >> 2]);
+          w[r] = acc;
         }
-        for (int g = tid; g < E / 16; g += NT) {
-          const int b = SWZ ? lds_swz<true>(lane_deposit(g, op.lanes, M - 2)) : ins_zero(ins_zero(g, jl), jh);
-          T v[4][4];
 #pragma unroll
-          for (int c = 0; c < 4; ++c)
-#pragma unroll
-            for (int t = 0; t < 4; ++t) v[c][t] = s[SWZ ? (b ^ kc[c] ^ kt[t]) : (b + kc[c] + kt[t])];
-          if (!GEN2Q || op.kind == TOP_CX) {
-            tile_2q<T, false>(v, op, pv);
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-              for (int t = 0; t < 4; ++t)
-                if (c | t) s[SWZ ? (b ^ kc[c] ^ kt[t]) : (b + kc[c] + kt[t])] = v[c][t];
-          } else {
-            // 16x16 PTM: each output row straight to LDS (no second buffer)
-#pragma unroll
-            for (int r = 1; r < 16; ++r) {
-              T acc = T(0);
-#pragma unroll
-              for (int c = 0; c < 16; ++c) acc += pv[16 * r + c] * v[c >> 2][c & 3];
-              s[SWZ ? (b ^ kc[r >> 2] ^ kt[r & 3]) : (b + kc[r >> 2] + kt[r & 3])] = acc;
-            }
-          }
+        for (int r = 1; r < 16; ++r) {
+          if (lo_first) v[g][r >> 2][r & 3] = w[r];
+          else v[g][r & 3][r >> 2] = w[r];
         }
       }
-    } else if (DENSE && op.kind == TOP_DENSE) {
-      if (M == 6) tile_dense<T, M>(s, op, pv);
+    }
+  }
+}
+
+// The operations of a tile program on the LDS tile s (all threads of the
+// workgroup take part; a barrier follows each operation).  Every operation
+// is a stage (TOP_STAGE, digits ja < jb) or, in DENSE variants, a matrix-core
+// PTM.  A stage reads each group of 16 coefficients (the 4 x 4 components
+// along ja and jb) once, applies its members in registers and writes the 15
+// that can change (the (I, I) component never does).  Access forms:
+//  - ja = 0: one quad (16 bytes along digit 0) per component of jb;
+//  - VEC, FP32, ja >= 1: the two groups that differ in the low bit of digit
+//    0, one 8-byte access per component pair;
+//  - otherwise one 4- or 8-byte access per coefficient.
+// SWZ: the tile is stored at lds_swz of each tile index, and a stage
+// enumerates its groups in the digit order of TileOp::lanes (quad and
+// element forms) or TileOp::qlanes (8-byte form), so that the lanes of a
+// wave fall on distinct banks.  VEC is off in the 64-register one-tile
+// kernel of 6-digit tiles (measured on MI355X: the 8-byte form's registers
+// spill there).
+template <typename T, int M, bool DENSE, bool GEN2Q, int NT, bool SWZ, bool VEC = true>
+__device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, int nops, const T *__restrict__ prm) {
+  constexpr int E = 1 << (2 * M);
+  using Q = Pack<T, 4>;
+  const int tid = threadIdx.x;
+  for (int i = 0; i < nops; ++i) {
+    const TileOp op = ops[i];
+    if (DENSE && op.kind == TOP_DENSE) {
+      if (M == 6) tile_dense<T, M>(s, op, prm + op.param);
+      __syncthreads();
+      continue;
+    }
+    // TOP_STAGE: the members follow the header in the list
+    const int ja = op.d[0], jb = op.d[1];
+    const TileOp *mem = ops + i + 1;
+    const int nm = op.k;
+    i += nm;
+    int ka[4], kb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      ka[u] = lds_swz<SWZ>(u << (2 * ja));
+      kb[u] = lds_swz<SWZ>(u << (2 * jb));
+    }
+    if (ja == 0) {
+      for (int g = tid; g < E / 16; g += NT) {
+        const int b = SWZ ? lds_swz<true>(lane_deposit(g, op.lanes, M - 2)) : ins_zero(ins_zero(g, 0), jb);
+        T v[1][4][4];
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+          const Q qd = *reinterpret_cast<const Q *>(&s[b ^ kb[y]]);
+#pragma unroll
+          for (int x = 0; x < 4; ++x) v[0][x][y] = qd.v[x];
+        }
+        stage_apply<T, GEN2Q, 1>(v, 0, mem, nm, prm);
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+          Q qd;
+#pragma unroll
+          for (int x = 0; x < 4; ++x) qd.v[x] = v[0][x][y];
+          *reinterpret_cast<Q *>(&s[b ^ kb[y]]) = qd;
+        }
+      }
+    } else if (VEC && sizeof(T) == 4) {
+      using P2 = Pack<T, 2>;
+      constexpr int UN = E / 32;
+      for (int un = tid; un < UN; un += NT) {
+        const int b = lds_swz<SWZ>(((un & 1) << 1) | lane_deposit(un >> 1, op.qlanes, M - 3));
+        T v[2][4][4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+          for (int y = 0; y < 4; ++y) {
+            const P2 p = *reinterpret_cast<const P2 *>(&s[b ^ ka[x] ^ kb[y]]);
+            v[0][x][y] = p.v[0];
+            v[1][x][y] = p.v[1];
+          }
+        stage_apply<T, GEN2Q, 2>(v, ja, mem, nm, prm);
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+          for (int y = 0; y < 4; ++y)
+            if (x | y) {
+              P2 p;
+              p.v[0] = v[0][x][y];
+              p.v[1] = v[1][x][y];
+              *reinterpret_cast<P2 *>(&s[b ^ ka[x] ^ kb[y]]) = p;
+            }
+      }
+    } else {
+      for (int g = tid; g < E / 16; g += NT) {
+        const int b = SWZ ? lds_swz<true>(lane_deposit(g, op.lanes, M - 2)) : ins_zero(ins_zero(g, ja), jb);
+        T v[1][4][4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+          for (int y = 0; y < 4; ++y) v[0][x][y] = s[b ^ ka[x] ^ kb[y]];
+        stage_apply<T, GEN2Q, 1>(v, ja, mem, nm, prm);
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+          for (int y = 0; y < 4; ++y)
+            if (x | y) s[b ^ ka[x] ^ kb[y]] = v[0][x][y];
+      }
     }
     __syncthreads();
   }
@@ -1023,16 +966,28 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
   };
   // quad c = tid + i * NT is tile indices 4c..4c+3, 4 consecutive
   // coefficients in HBM and (swizzled or not) in LDS
+  // one workgroup per tile (remap = 0) reads runs of >= 4^3 coefficients:
+  // nontemporal loads and stores; persistent grids share HBM lines between
+  // neighbouring tiles through L2
+  const bool nt = remap == 0;
   Q buf[QPT];
-  uint64_t tile = lid;
-  if (tile < geo.tiles) {
-    const uint64_t base = tile_base(tile);
+  auto load_tile = [&](uint64_t b) {
+    if (nt) {
 #pragma unroll
-    for (int i = 0; i < QPT; ++i) {
-      const int c = tid + i * NT;
-      if (c < E / 4) buf[i] = ldp<Q>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo));
+      for (int i = 0; i < QPT; ++i) {
+        const int c = tid + i * NT;
+        if (c < E / 4) buf[i] = ldq<T, true>(a + b + tile_offset<T, M>(4 * c, w, lomask, geo));
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < QPT; ++i) {
+        const int c = tid + i * NT;
+        if (c < E / 4) buf[i] = ldq<T, false>(a + b + tile_offset<T, M>(4 * c, w, lomask, geo));
+      }
     }
-  }
+  };
+  uint64_t tile = lid;
+  if (tile < geo.tiles) load_tile(tile_base(tile));
   for (; tile < geo.tiles; tile += gridDim.x) {
     const uint64_t base = tile_base(tile);
 #pragma unroll
@@ -1042,32 +997,36 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
     }
     // prefetch the next tile while this one is computed
     const uint64_t next = tile + gridDim.x;
-    if (next < geo.tiles) {
-      const uint64_t nb = tile_base(next);
+    if (next < geo.tiles) load_tile(tile_base(next));
+    __syncthreads();
+    tile_ops<T, M, DENSE, GEN2Q, NT, SWZ>(s, ops, nops, prm);
+    if (nt) {
 #pragma unroll
       for (int i = 0; i < QPT; ++i) {
         const int c = tid + i * NT;
-        if (c < E / 4) buf[i] = ldp<Q>(a + nb + tile_offset<T, M>(4 * c, w, lomask, geo));
+        if (c < E / 4) stq<T, true>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), lds_get_quad(s, lds_swz<SWZ>(4 * c)));
       }
-    }
-    __syncthreads();
-    tile_ops<T, M, DENSE, GEN2Q, NT, SWZ>(s, ops, nops, prm);
+    } else {
 #pragma unroll
-    for (int i = 0; i < QPT; ++i) {
-      const int c = tid + i * NT;
-      if (c < E / 4) stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), lds_get_quad(s, lds_swz<SWZ>(4 * c)));
+      for (int i = 0; i < QPT; ++i) {
+        const int c = tid + i * NT;
+        if (c < E / 4) stq<T, false>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), lds_get_quad(s, lds_swz<SWZ>(4 * c)));
+      }
     }
     __syncthreads();
   }
 }
 
-// One tile per workgroup (grid = tiles, tile = block index): no prefetch
-// buffer, so fewer registers and more resident workgroups per CU, which
-// overlap each other's HBM and LDS phases.  FP32: 8 waves per SIMD (64
+// One tile per workgroup (grid = tiles, tile = block index): no prefetch of
+// a next tile, so fewer registers during the operations and more resident
+// workgroups per CU, which overlap each other's HBM and LDS phases.  Runs
+// are >= 4^3 coefficients: nontemporal loads and stores (measured faster on
+// MI355X for such runs, 6 % on 7-digit tiles).  FP32: 8 waves per SIMD (64
 // VGPRs), 7 for swizzled tiles (72 VGPRs: their XOR addresses cannot use
-// instruction offsets; 64 spilled).
+// instruction offsets; 64 spilled); 7-digit tiles 4 waves (128 VGPRs), the
+// two 512-thread workgroups their 64 KB of LDS allow per CU.
 template <typename T, int M, bool GEN2Q, int NT, bool SWZ>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (SWZ ? 7 : 8) : 1))) void k_tile_stream(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (M == 7 ? 4 : SWZ ? 7 : 8) : 1))) void k_tile_stream(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
                                                         int nops, const T *__restrict__ prm) {
   constexpr int E = 1 << (2 * M);
   constexpr int QPT = (E / 4 + NT - 1) / NT;
@@ -1083,17 +1042,49 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
     base |= (r & 3ull) << (2 * geo.rpos[j]);
     r >>= 2;
   }
+  if constexpr (M >= 7) {
+    // 2 workgroups per CU: every load of the tile is issued before the first
+    // LDS write (and every LDS read before the first store), so a thread
+    // waits for HBM once.  6-digit tiles (8 workgroups per CU, 64 VGPRs)
+    // keep one quad in flight per thread, which needs no buffer registers.
+    Q buf[QPT];
 #pragma unroll
-  for (int i = 0; i < QPT; ++i) {
-    const int c = tid + i * NT;
-    if (c < E / 4) lds_put_quad(s, lds_swz<SWZ>(4 * c), ldp<Q>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo)));
+    for (int i = 0; i < QPT; ++i) {
+      const int c = tid + i * NT;
+      if (c < E / 4) buf[i] = ldq<T, true>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo));
+    }
+#pragma unroll
+    for (int i = 0; i < QPT; ++i) {
+      const int c = tid + i * NT;
+      if (c < E / 4) lds_put_quad(s, lds_swz<SWZ>(4 * c), buf[i]);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < QPT; ++i) {
+      const int c = tid + i * NT;
+      if (c < E / 4) lds_put_quad(s, lds_swz<SWZ>(4 * c), ldq<T, true>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo)));
+    }
   }
   __syncthreads();
   tile_ops<T, M, false, GEN2Q, NT, SWZ, (M >= 7)>(s, ops, nops, prm);
+  if constexpr (M >= 7) {
+    Q buf[QPT];
 #pragma unroll
-  for (int i = 0; i < QPT; ++i) {
-    const int c = tid + i * NT;
-    if (c < E / 4) stp(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), lds_get_quad(s, lds_swz<SWZ>(4 * c)));
+    for (int i = 0; i < QPT; ++i) {
+      const int c = tid + i * NT;
+      if (c < E / 4) buf[i] = lds_get_quad(s, lds_swz<SWZ>(4 * c));
+    }
+#pragma unroll
+    for (int i = 0; i < QPT; ++i) {
+      const int c = tid + i * NT;
+      if (c < E / 4) stq<T, true>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), buf[i]);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < QPT; ++i) {
+      const int c = tid + i * NT;
+      if (c < E / 4) stq<T, true>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), lds_get_quad(s, lds_swz<SWZ>(4 * c)));
+    }
   }
 }
 
@@ -2202,6 +2193,94 @@ int validate_op(const dm_state *h, const dm_op &op, int i, int64_t nparams) {
   return DM_OK;
 }
 
+// Stages.  An operation applied on its own costs one LDS round trip of the
+// whole tile; operations whose digits fit in two tile digits are gathered
+// into stages that apply them in registers, one round trip per stage.  An
+// operation joins the first stage, at or after the last stage that touches
+// its digits, whose digit set stays within two digits (it commutes with
+// every later stage, which acts on other digits); otherwise it opens a new
+// stage at the end.  Single-qubit operations that end up alone on their
+// digit become one product PTM, run as a stage with digit 0 (or digit 3 when
+// the digit is 0).  A dense operation is a stage of its own and stays a
+// TOP_DENSE record.  Every other record of the result is a TOP_STAGE header
+// followed by its members.
+void group_stages(FusedPass *fp) {
+  struct Stage {
+    unsigned mask;
+    std::vector<int> mem;
+    bool dense;
+  };
+  std::vector<Stage> st;
+  const int nops = (int)fp->ops.size();
+  for (int i = 0; i < nops; ++i) {
+    const TileOp &op = fp->ops[i];
+    const int nd = op.kind == TOP_1Q ? 1 : op.kind == TOP_DENSE ? op.k : 2;
+    unsigned D = 0;
+    for (int x = 0; x < nd; ++x) D |= 1u << op.d[x];
+    int j = -1;
+    for (int k = 0; k < (int)st.size(); ++k)
+      if (st[k].mask & D) j = k;
+    int best = -1;
+    if (op.kind != TOP_DENSE)
+      for (int k = std::max(j, 0); k < (int)st.size() && best < 0; ++k)
+        if (!st[k].dense && __builtin_popcount(st[k].mask | D) <= 2) best = k;
+    if (best < 0) {
+      st.push_back(Stage{D, {i}, op.kind == TOP_DENSE});
+    } else {
+      st[best].mask |= D;
+      st[best].mem.push_back(i);
+    }
+  }
+  std::vector<TileOp> out;
+  out.reserve(nops + st.size());
+  for (const Stage &s : st) {
+    if (s.dense) {
+      out.push_back(fp->ops[s.mem[0]]);
+      continue;
+    }
+    TileOp hd;
+    memset(&hd, 0, sizeof(hd));
+    hd.kind = TOP_STAGE;
+    if (__builtin_popcount(s.mask) == 1) {
+      // P = R_last ... R_first over the components (I, X, Y, Z); each R has
+      // the identity's first row and rows X, Y, Z from the operation
+      TileOp op = fp->ops[s.mem[0]];
+      if (s.mem.size() > 1) {
+        double P[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+        for (int i : s.mem) {
+          double R[4][4] = {{1, 0, 0, 0}};
+          const double *r = &fp->vals[fp->ops[i].param];
+          for (int c = 0; c < 3; ++c)
+            for (int y = 0; y < 4; ++y) R[c + 1][y] = r[4 * c + y];
+          double Q[4][4];
+          for (int x = 0; x < 4; ++x)
+            for (int y = 0; y < 4; ++y) {
+              double acc = 0.0;
+              for (int z = 0; z < 4; ++z) acc += R[x][z] * P[z][y];
+              Q[x][y] = acc;
+            }
+          memcpy(P, Q, sizeof(P));
+        }
+        op.param = (int32_t)fp->vals.size();
+        for (int c = 1; c < 4; ++c)
+          for (int y = 0; y < 4; ++y) fp->vals.push_back(P[c][y]);
+      }
+      const int d = op.d[0];
+      hd.k = 1;
+      hd.d[0] = 0;
+      hd.d[1] = d == 0 ? 3 : d;
+      out.push_back(hd);
+      out.push_back(op);
+      continue;
+    }
+    hd.k = (int32_t)s.mem.size();
+    hd.d[0] = __builtin_ctz(s.mask);
+    hd.d[1] = 31 - __builtin_clz(s.mask);
+    out.push_back(hd);
+    for (int i : s.mem) out.push_back(fp->ops[i]);
+  }
+  fp->ops.swap(out);
+}
 
 int parse_fused(const dm_state *h, const double *p, int64_t avail, int opi, FusedPass *fp) {
   int64_t at = 0;
@@ -2300,30 +2379,33 @@ int parse_fused(const dm_state *h, const double *p, int64_t avail, int opi, Fuse
     fp->ops.push_back(op);
   }
   // LDS layout.  The 64 banks (4 bytes each) are LDS index bits 0-5, tile
-  // digits 0-2.  Operations on digits 3 and up, and single-qubit operations
-  // on digit 0 (one vector access per group), fall on distinct banks in the
-  // identity layout.  A pass with a two-qubit operation on digits 0-2, or a
-  // single-qubit operation on digit 1 or 2, whose lanes would hit each bank
+  // digits 0-2.  Stages on digits 3 and up, and stages on digit 0 and a
+  // digit >= 3 (one quad access per component), fall on distinct banks in
+  // the identity layout.  A pass with any other stage (digit 1 or 2 without
+  // digit 0, or digit 0 with digit 1 or 2), whose lanes would hit each bank
   // 4 or 16 times (measured on MI355X: a CX on digits 0 and 1 costs 3.5x a
   // CX on digits 3 and 4), stores the tile swizzled (lds_swz, bank fields
-  // [d0, d1^d3, d2^d4]) and gives each operation an order of its free digits
+  // [d0, d1^d3, d2^d4]) and gives each stage an order of its free digits
   // whose first ones cover distinct fields: d0, then d1 (else d3), then d2
-  // (else d4), then the rest ascending.  A two-qubit operation on digit 0
-  // accesses whole quads; its first free digits cover fields 1 and 2.
-  // Matrix-core passes keep the identity layout.
+  // (else d4), then the rest ascending.  Matrix-core passes keep the
+  // identity layout.
+  group_stages(fp);
   bool dense = false, swz = false;
-  for (const TileOp &op : fp->ops) {
+  for (size_t i = 0; i < fp->ops.size(); ++i) {
+    const TileOp &op = fp->ops[i];
     dense = dense || op.kind == TOP_DENSE;
-    if (op.kind == TOP_1Q) swz = swz || op.d[0] == 1 || op.d[0] == 2;
-    if (op.kind == TOP_CX || op.kind == TOP_2Q) swz = swz || op.d[0] < 3 || op.d[1] < 3;
+    if (op.kind != TOP_STAGE) continue;
+    swz = swz || (op.d[0] == 0 ? op.d[1] < 3 : op.d[0] < 3);
+    i += op.k;  // members run in registers
   }
   swz = swz && !dense;
   fp->geo.ident = swz ? 0 : 1;
+  // group orders of the stage headers (members run in registers)
   if (swz)
     for (TileOp &op : fp->ops) {
-      const int nd = op.kind == TOP_1Q ? 1 : 2;
+      if (op.kind != TOP_STAGE) continue;
       bool busy[8] = {false};
-      for (int x = 0; x < nd; ++x) busy[op.d[x]] = true;
+      busy[op.d[0]] = busy[op.d[1]] = true;
       int ord[8], nf = 0;
       auto take = [&](int d) {
         if (d < m && !busy[d]) {
@@ -2342,11 +2424,9 @@ int parse_fused(const dm_state *h, const double *p, int64_t avail, int opi, Fuse
       op.lanes = packed;
     }
   for (TileOp &op : fp->ops) {
-    if (op.kind == TOP_DENSE) continue;
-    const int nd = op.kind == TOP_1Q ? 1 : 2;
+    if (op.kind != TOP_STAGE) continue;
     bool busy[8] = {false};
-    busy[0] = true;
-    for (int x = 0; x < nd; ++x) busy[op.d[x]] = true;
+    busy[0] = busy[op.d[0]] = busy[op.d[1]] = true;
     int ord[8], nf = 0;
     auto take = [&](int d) {
       if (d < m && !busy[d]) {

@@ -413,6 +413,56 @@ def test_fused_pass_two_qubit_ops_on_low_digits(prec, m, qs):
     assert np.max(np.abs(got - want)) <= t
 
 
+@pytest.mark.parametrize('prec', PRECS)
+@pytest.mark.parametrize('m,seed', [(5, 1), (6, 2), (6, 3), (7, 4), (7, 5)])
+def test_fused_pass_stages_of_random_operations(prec, m, seed):
+    """A pass of 48 random operations on 4 qubits of the tile (single-qubit
+    PTMs, scaled and plain CX, dense two-qubit PTMs, repeats on one qubit
+    between operations on others): the library regroups them into
+    two-digit register stages, moving operations past commuting ones and
+    multiplying lone single-qubit runs; results must equal one kernel per
+    operation in the given order."""
+    from dmsim.program import Program
+    n = 9
+    rng = np.random.default_rng(100 + seed)
+    a = rand_state(n, rng)
+    tile_qs = [int(q) for q in rng.choice(n, size=m if m < 7 else 6, replace=False)]
+    act = tile_qs[:4]
+    sub, plain = [], Program(n)
+    for _ in range(48):
+        r = rng.random()
+        if r < 0.45:
+            q = int(rng.choice(act))
+            P = ptm.depolarizing1(1e-3) @ ptm.gate_ptm('u3', list(rng.uniform(-3, 3, 3)))
+            sub.append((native.SUB_1Q, [q], ptm.rows3(P)))
+            plain.one_qubit(q, P)
+        else:
+            c, t = (int(x) for x in rng.choice(act, size=2, replace=False))
+            if r < 0.65:
+                S = ptm.depolarizing_pair_scale(float(rng.uniform(1e-4, 1e-2)))
+                sub.append((native.SUB_CX, [c, t], S))
+                plain.cx(c, t, S)
+            elif r < 0.8:
+                sub.append((native.SUB_CX, [c, t], None))
+                plain.cx(c, t)
+            else:
+                T = ptm.cx_ptm((0.97, float(rng.uniform(0, 0.1))))
+                sub.append((native.SUB_2Q, [c, t], T))
+                plain.two_qubit(c, t, T)
+    fused = Program(n)
+    fused.fused(m, tile_qs, sub)
+    s1 = make(n, 'fp64', a)
+    plain.run(s1)
+    want = s1.download()
+    s1.close()
+    s2 = make(n, prec, a)
+    fused.run(s2)
+    got = s2.download()
+    s2.close()
+    t = 1e-14 * 2.0 ** -n * 50 if prec == 'fp64' else 2e-6 * 2.0 ** -n * 50
+    assert np.max(np.abs(got - want)) <= t
+
+
 def test_fused_pass_rejects_bad_records():
     from dmsim.program import Program
     st = native.DeviceState(6, 'fp64')
