@@ -937,12 +937,13 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
 // A fused pass: each workgroup loads a tile (the M tile digits of geo) into
 // LDS, applies the pass's operations and writes it back, looping over tiles
 // with stride gridDim.x and prefetching its next tile into registers.  The
-// host launches grid = tiles (remap = 0) or a persistent grid (remap = 1,
-// consecutive tiles kept on one XCD).  SWZ: the tile sits in LDS at lds_swz
-// of each index.
+// host launches grid = tiles or a persistent grid; mode bit 0 (persistent
+// grids over runs of 4^2) keeps consecutive tiles on one XCD, bit 1 (runs of
+// >= 4^3) makes the tile loads and stores nontemporal.  SWZ: the tile sits
+// in LDS at lds_swz of each index.
 template <typename T, int M, bool DENSE, bool GEN2Q, int NT, bool SWZ>
 __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
-                                                      int nops, const T *__restrict__ prm, int remap) {
+                                                      int nops, const T *__restrict__ prm, int mode) {
   constexpr int E = 1 << (2 * M);
   constexpr int QPT = (E / 4 + NT - 1) / NT;  // quads per thread (4 for M=6, 1 for M=5)
   using Q = Pack<T, 4>;
@@ -955,6 +956,7 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
   // neighbouring tiles share HBM lines) consecutive tiles go to workgroups of
   // the same XCD: logical id = (b % 8) * (grid / 8) + b / 8.
   const uint64_t g8 = gridDim.x / 8;
+  const bool remap = mode & 1;
   const uint64_t lid = (remap && gridDim.x % 8 == 0) ? (blockIdx.x % 8) * g8 + blockIdx.x / 8 : blockIdx.x;
   auto tile_base = [&](uint64_t tile) {
     uint64_t base = 0, r = tile;
@@ -966,10 +968,9 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
   };
   // quad c = tid + i * NT is tile indices 4c..4c+3, 4 consecutive
   // coefficients in HBM and (swizzled or not) in LDS
-  // one workgroup per tile (remap = 0) reads runs of >= 4^3 coefficients:
-  // nontemporal loads and stores; persistent grids share HBM lines between
-  // neighbouring tiles through L2
-  const bool nt = remap == 0;
+  // runs of >= 4^3 coefficients: nontemporal loads and stores; runs of 4^2
+  // share HBM lines between neighbouring tiles through L2
+  const bool nt = mode & 2;
   Q buf[QPT];
   auto load_tile = [&](uint64_t b) {
     if (nt) {
@@ -1026,7 +1027,7 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
 // instruction offsets; 64 spilled); 7-digit tiles 4 waves (128 VGPRs), the
 // two 512-thread workgroups their 64 KB of LDS allow per CU.
 template <typename T, int M, bool GEN2Q, int NT, bool SWZ>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (M == 7 ? 4 : SWZ ? 7 : 8) : 1))) void k_tile_stream(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? ((1 << (2 * M)) / NT >= 32 ? 4 : SWZ && NT <= 256 ? 7 : 8) : 1))) void k_tile_stream(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
                                                         int nops, const T *__restrict__ prm) {
   constexpr int E = 1 << (2 * M);
   constexpr int QPT = (E / 4 + NT - 1) / NT;
@@ -1042,7 +1043,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
     base |= (r & 3ull) << (2 * geo.rpos[j]);
     r >>= 2;
   }
-  if constexpr (M >= 7) {
+  if constexpr (E / NT >= 32) {
     // 2 workgroups per CU: every load of the tile is issued before the first
     // LDS write (and every LDS read before the first store), so a thread
     // waits for HBM once.  6-digit tiles (8 workgroups per CU, 64 VGPRs)
@@ -1066,8 +1067,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
     }
   }
   __syncthreads();
-  tile_ops<T, M, false, GEN2Q, NT, SWZ, (M >= 7)>(s, ops, nops, prm);
-  if constexpr (M >= 7) {
+  tile_ops<T, M, false, GEN2Q, NT, SWZ, (E / NT >= 32)>(s, ops, nops, prm);
+  if constexpr (E / NT >= 32) {
     Q buf[QPT];
 #pragma unroll
     for (int i = 0; i < QPT; ++i) {
@@ -2502,6 +2503,10 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
   //    one resident workgroup per CU): persistent workgroups that prefetch
   //    their next tile, 8 (resp. 4) per resident slot, with consecutive
   //    tiles kept on one XCD.
+  // Measured and rejected for FP32 7-digit tiles over runs of 4^3: the
+  // persistent prefetching kernel (one resident workgroup per CU: 1.56x the
+  // headline pass time) and one-tile workgroups of 1024 threads (64 VGPRs,
+  // 8 waves per SIMD: 1.36x).
 #define TILE_LAUNCH_L(MM, D, G, NTH, L)                                                                          \
   do {                                                                                                           \
     const int per_cu = tile_resident_per_cu<T, MM, D, G, NTH, L>();                                             \
@@ -2518,7 +2523,7 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
     uint64_t g = full ? fp.geo.tiles : std::min<uint64_t>(fp.geo.tiles, slots);                                 \
     if (!full && g >= 8) g -= g % 8;                                                                             \
     hipLaunchKernelGGL((k_tile_prog<T, MM, D, G, NTH, L>), dim3((unsigned)g), dim3(NTH), 0, h->stream, a,       \
-                       fp.geo, d_ops, nops, d_vals, full ? 0 : 1);                                               \
+                       fp.geo, d_ops, nops, d_vals, (full ? 0 : 1) | (fp.geo.w >= 3 ? 2 : 0));                   \
   } while (0)
 #define TILE_LAUNCH(MM, D, G, NTH)               \
   do {                                           \
