@@ -938,9 +938,9 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
 // LDS, applies the pass's operations and writes it back, looping over tiles
 // with stride gridDim.x and prefetching its next tile into registers.  The
 // host launches grid = tiles or a persistent grid; mode bit 0 (persistent
-// grids over runs of 4^2) keeps consecutive tiles on one XCD, bit 1 (runs of
-// >= 4^3) makes the tile loads and stores nontemporal.  SWZ: the tile sits
-// in LDS at lds_swz of each index.
+// grids) keeps consecutive tiles on one XCD, bit 1 (FP32 runs of >= 4^3)
+// makes the tile loads and stores nontemporal.  SWZ: the tile sits in LDS
+// at lds_swz of each index.
 template <typename T, int M, bool DENSE, bool GEN2Q, int NT, bool SWZ>
 __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
                                                       int nops, const T *__restrict__ prm, int mode) {
@@ -968,8 +968,8 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
   };
   // quad c = tid + i * NT is tile indices 4c..4c+3, 4 consecutive
   // coefficients in HBM and (swizzled or not) in LDS
-  // runs of >= 4^3 coefficients: nontemporal loads and stores; runs of 4^2
-  // share HBM lines between neighbouring tiles through L2
+  // nontemporal loads and stores (FP32 runs of >= 4^3 coefficients); runs
+  // of 4^2 share HBM lines between neighbouring tiles through L2
   const bool nt = mode & 2;
   Q buf[QPT];
   auto load_tile = [&](uint64_t b) {
@@ -2503,6 +2503,8 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
   //    one resident workgroup per CU): persistent workgroups that prefetch
   //    their next tile, 8 (resp. 4) per resident slot, with consecutive
   //    tiles kept on one XCD.
+  // FP32 tiles over runs of >= 4^3 load and store nontemporally (FP64
+  // passes measured 1.3x slower with the hint).
   // Measured and rejected for FP32 7-digit tiles over runs of 4^3: the
   // persistent prefetching kernel (one resident workgroup per CU: 1.56x the
   // headline pass time) and one-tile workgroups of 1024 threads (64 VGPRs,
@@ -2523,7 +2525,7 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
     uint64_t g = full ? fp.geo.tiles : std::min<uint64_t>(fp.geo.tiles, slots);                                 \
     if (!full && g >= 8) g -= g % 8;                                                                             \
     hipLaunchKernelGGL((k_tile_prog<T, MM, D, G, NTH, L>), dim3((unsigned)g), dim3(NTH), 0, h->stream, a,       \
-                       fp.geo, d_ops, nops, d_vals, (full ? 0 : 1) | (fp.geo.w >= 3 ? 2 : 0));                   \
+                       fp.geo, d_ops, nops, d_vals, (full ? 0 : 1) | (sizeof(T) == 4 && fp.geo.w >= 3 ? 2 : 0)); \
   } while (0)
 #define TILE_LAUNCH(MM, D, G, NTH)               \
   do {                                           \
