@@ -1018,6 +1018,135 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
   }
 }
 
+// The first or last stage of a one-tile pass, fused with the HBM copy: DIR
+// 0 reads the stage's groups of 16 coefficients from HBM and writes its
+// result to the LDS tile, DIR 1 reads the groups from LDS and writes the
+// result to HBM, DIR 2 (a pass of one stage) goes from HBM to HBM without
+// LDS.  Saves one LDS round trip on either side.  Only for stages whose
+// group order is contiguous in HBM and conflict-free in LDS (stage_io_ok):
+// digit 0 with a digit >= 3 (one quad per component; lanes over digits 1, 2
+// first), or two digits >= 3 (lanes over digits 0-2 first; FP32 7-digit
+// tiles pair the groups that differ in the low bit of digit 0 into 8-byte
+// accesses).  A thread loads all its groups before the first operation.
+__host__ __device__ __forceinline__ bool stage_io_ok(const TileOp &op) {
+  return (op.d[0] == 0 && op.d[1] >= 3) || op.d[0] >= 3;
+}
+
+template <typename T, int M, bool GEN2Q, int NT, bool SWZ, bool VEC, int DIR>
+__device__ __forceinline__ void stage_io(T *__restrict__ a, uint64_t base, int w, int lomask, const TileGeom &geo,
+                                         T *s, const TileOp *__restrict__ ops, int at, const T *__restrict__ prm) {
+  constexpr int E = 1 << (2 * M);
+  constexpr int GPT = (E / 16 + NT - 1) / NT;  // groups per thread
+  using Q = Pack<T, 4>;
+  const TileOp op = ops[at];
+  const int ja = op.d[0], jb = op.d[1];
+  const TileOp *mem = ops + at + 1;
+  const int nm = op.k;
+  const int tid = threadIdx.x;
+  int ka[4], kb[4];
+  uint64_t ha[4], hb[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    ka[u] = lds_swz<SWZ>(u << (2 * ja));
+    kb[u] = lds_swz<SWZ>(u << (2 * jb));
+    ha[u] = (uint64_t)u << (2 * geo.tpos[ja]);
+    hb[u] = (uint64_t)u << (2 * geo.tpos[jb]);
+  }
+  if (VEC && sizeof(T) == 4 && ja != 0) {
+    // one unit per thread: the two groups that differ in the low bit of
+    // digit 0, one 8-byte access per component pair
+    using P2 = Pack<T, 2>;
+    const int un = tid;
+    if (un < E / 32) {
+      const int bt = ((un & 1) << 1) | lane_deposit(un >> 1, op.qlanes, M - 3);
+      const int lb = lds_swz<SWZ>(bt);
+      T *hp = a + base + tile_offset<T, M>(bt, w, lomask, geo);
+      T v[2][4][4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+          P2 p;
+          if (DIR == 1) {
+            p = *reinterpret_cast<const P2 *>(&s[lb ^ ka[x] ^ kb[y]]);
+          } else {
+            using V2 = T __attribute__((ext_vector_type(2)));
+            const V2 q = __builtin_nontemporal_load(reinterpret_cast<const V2 *>(hp + ha[x] + hb[y]));
+            p.v[0] = q[0];
+            p.v[1] = q[1];
+          }
+          v[0][x][y] = p.v[0];
+          v[1][x][y] = p.v[1];
+        }
+      stage_apply<T, GEN2Q, 2>(v, ja, mem, nm, prm);
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+          if (DIR == 0) {
+            P2 p;
+            p.v[0] = v[0][x][y];
+            p.v[1] = v[1][x][y];
+            *reinterpret_cast<P2 *>(&s[lb ^ ka[x] ^ kb[y]]) = p;
+          } else {
+            using V2 = T __attribute__((ext_vector_type(2)));
+            V2 q;
+            q[0] = v[0][x][y];
+            q[1] = v[1][x][y];
+            __builtin_nontemporal_store(q, reinterpret_cast<V2 *>(hp + ha[x] + hb[y]));
+          }
+        }
+    }
+    return;
+  }
+  int lb[GPT];
+  T *hp[GPT];
+  T v[GPT][4][4];
+#pragma unroll
+  for (int q = 0; q < GPT; ++q) {
+    const int g = tid + q * NT;
+    const int bt = SWZ ? lane_deposit(g, op.lanes, M - 2) : ins_zero(ins_zero(g, ja), jb);
+    lb[q] = lds_swz<SWZ>(bt);
+    hp[q] = a + base + tile_offset<T, M>(bt, w, lomask, geo);
+    if (ja == 0) {
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {
+        const Q qd = DIR == 1 ? lds_get_quad(s, lb[q] ^ kb[y]) : ldq<T, true>(hp[q] + hb[y]);
+#pragma unroll
+        for (int x = 0; x < 4; ++x) v[q][x][y] = qd.v[x];
+      }
+    } else {
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+          v[q][x][y] = DIR == 1 ? s[lb[q] ^ ka[x] ^ kb[y]] : __builtin_nontemporal_load(hp[q] + ha[x] + hb[y]);
+    }
+  }
+  stage_apply<T, GEN2Q, GPT>(v, ja, mem, nm, prm);
+#pragma unroll
+  for (int q = 0; q < GPT; ++q) {
+    if (ja == 0) {
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {
+        Q qd;
+#pragma unroll
+        for (int x = 0; x < 4; ++x) qd.v[x] = v[q][x][y];
+        if (DIR == 0) lds_put_quad(s, lb[q] ^ kb[y], qd);
+        else stq<T, true>(hp[q] + hb[y], qd);
+      }
+    } else {
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+          if (DIR == 0) s[lb[q] ^ ka[x] ^ kb[y]] = v[q][x][y];
+          else __builtin_nontemporal_store(v[q][x][y], hp[q] + ha[x] + hb[y]);
+        }
+    }
+  }
+}
+
 // One tile per workgroup (grid = tiles, tile = block index): no prefetch of
 // a next tile, so fewer registers during the operations and more resident
 // workgroups per CU, which overlap each other's HBM and LDS phases.  Runs
@@ -1028,7 +1157,7 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
 // two 512-thread workgroups their 64 KB of LDS allow per CU.
 template <typename T, int M, bool GEN2Q, int NT, bool SWZ>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? ((1 << (2 * M)) / NT >= 32 ? 4 : SWZ && NT <= 256 ? 7 : 8) : 1))) void k_tile_stream(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
-                                                        int nops, const T *__restrict__ prm) {
+                                                        int nops, const T *__restrict__ prm, int io, int last_at) {
   constexpr int E = 1 << (2 * M);
   constexpr int QPT = (E / 4 + NT - 1) / NT;
   using Q = Pack<T, 4>;
@@ -1043,7 +1172,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
     base |= (r & 3ull) << (2 * geo.rpos[j]);
     r >>= 2;
   }
-  if constexpr (E / NT >= 32) {
+  // io bit 0: the first operation is a stage that reads HBM itself, bit 1:
+  // the stage with header last_at (the last operation) writes HBM itself
+  constexpr bool VEC = E / NT >= 32;
+  if ((io & 3) == 3 && last_at == 0) {
+    stage_io<T, M, GEN2Q, NT, SWZ, VEC, 2>(a, base, w, lomask, geo, s, ops, 0, prm);
+    return;
+  }
+  int i0 = 0;
+  if (io & 1) {
+    stage_io<T, M, GEN2Q, NT, SWZ, VEC, 0>(a, base, w, lomask, geo, s, ops, 0, prm);
+    i0 = 1 + ops[0].k;
+  } else if constexpr (VEC) {
     // 2 workgroups per CU: every load of the tile is issued before the first
     // LDS write (and every LDS read before the first store), so a thread
     // waits for HBM once.  6-digit tiles (8 workgroups per CU, 64 VGPRs)
@@ -1067,8 +1207,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
     }
   }
   __syncthreads();
-  tile_ops<T, M, false, GEN2Q, NT, SWZ, (E / NT >= 32)>(s, ops, nops, prm);
-  if constexpr (E / NT >= 32) {
+  const int i1 = (io & 2) ? last_at : nops;
+  tile_ops<T, M, false, GEN2Q, NT, SWZ, VEC>(s, ops + i0, i1 - i0, prm);
+  if (io & 2) {
+    stage_io<T, M, GEN2Q, NT, SWZ, VEC, 1>(a, base, w, lomask, geo, s, ops, last_at, prm);
+  } else if constexpr (VEC) {
     Q buf[QPT];
 #pragma unroll
     for (int i = 0; i < QPT; ++i) {
@@ -2486,6 +2629,13 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
     gen = gen || op.kind == TOP_2Q;
   }
   T *a = static_cast<T *>(h->d_a);
+  // the one-tile kernel fuses the first and the last stage with the HBM
+  // copy where their group order suits both HBM and LDS (stage_io_ok): io
+  // bits 0 and 1, last_stage the header index of the last operation
+  int last_stage = 0, io = 0;
+  for (int i = 0; i < nops; i += fp.ops[i].kind == TOP_STAGE ? 1 + fp.ops[i].k : 1) last_stage = i;
+  if (nops > 0 && fp.ops[0].kind == TOP_STAGE && stage_io_ok(fp.ops[0])) io |= 1;
+  if (nops > 0 && fp.ops[last_stage].kind == TOP_STAGE && stage_io_ok(fp.ops[last_stage])) io |= 2;
   // one variant per feature set keeps the register footprint of the common
   // 1q + CX passes small; 7-qubit tiles (64 / 128 KB of LDS) use 512 threads
   // Grid shape, from 16-qubit sweeps on MI355X:
@@ -2517,7 +2667,7 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
     const int s_cu = D ? 0 : stream_resident_per_cu<T, MM, G, NTH, L>();                                        \
     if (long_runs && !D && sizeof(T) == 4 && (s_cu * NTH / 64 >= 28 || (per_cu < 2 && s_cu >= 2))) {            \
       hipLaunchKernelGGL((k_tile_stream<T, MM, G, NTH, L>), dim3((unsigned)fp.geo.tiles), dim3(NTH), 0,          \
-                         h->stream, a, fp.geo, d_ops, nops, d_vals);                                             \
+                         h->stream, a, fp.geo, d_ops, nops, d_vals, io, last_stage);                             \
       break;                                                                                                     \
     }                                                                                                            \
     const bool full = long_runs && per_cu >= 2;                                                                  \
