@@ -837,8 +837,9 @@ __device__ __forceinline__ void stage_apply(T (&v)[NG][4][4], int a, const TileO
 // along ja and jb) once, applies its members in registers and writes the 15
 // that can change (the (I, I) component never does).  Access forms:
 //  - ja = 0: one quad (16 bytes along digit 0) per component of jb;
-//  - VEC, FP32, ja >= 1: the two groups that differ in the low bit of digit
-//    0, one 8-byte access per component pair;
+//  - VEC, FP32, ja >= 1, at least 32 coefficients per thread (7-digit tiles
+//    of 512 threads): the two groups that differ in the low bit of digit 0,
+//    one 8-byte access per component pair;
 //  - otherwise one 4- or 8-byte access per coefficient.
 // SWZ: the tile is stored at lds_swz of each tile index, and a stage
 // enumerates its groups in the digit order of TileOp::lanes (quad and
@@ -888,7 +889,9 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
           *reinterpret_cast<Q *>(&s[b ^ kb[y]]) = qd;
         }
       }
-    } else if (VEC && sizeof(T) == 4) {
+    } else if (VEC && sizeof(T) == 4 && E / NT >= 32) {
+      // (with fewer than 32 coefficients per thread the 8-byte form would
+      // leave threads idle)
       using P2 = Pack<T, 2>;
       constexpr int UN = E / 32;
       for (int un = tid; un < UN; un += NT) {
