@@ -2378,6 +2378,44 @@ void group_stages(FusedPass *fp) {
       st[best].mem.push_back(i);
     }
   }
+  // The one-tile kernel fuses the first and the last stage with the HBM copy
+  // when their digits allow it (stage_io_ok).  A stage that shares no digit
+  // with any stage before it may move to the front, one that shares none
+  // with any stage after it to the back (it commutes with the others).
+  auto io_ok = [&](const Stage &x) {
+    if (x.dense) return false;
+    TileOp hd;
+    memset(&hd, 0, sizeof(hd));
+    if (__builtin_popcount(x.mask) == 1) {
+      const int d = __builtin_ctz(x.mask);
+      hd.d[1] = d == 0 ? 3 : d;
+    } else {
+      hd.d[0] = __builtin_ctz(x.mask);
+      hd.d[1] = 31 - __builtin_clz(x.mask);
+    }
+    return stage_io_ok(hd);
+  };
+  const int ns = (int)st.size();
+  if (ns > 1 && !io_ok(st[0])) {
+    unsigned before = 0;
+    for (int k = 0; k < ns; ++k) {
+      if (!(st[k].mask & before) && io_ok(st[k])) {
+        std::rotate(st.begin(), st.begin() + k, st.begin() + k + 1);
+        break;
+      }
+      before |= st[k].mask;
+    }
+  }
+  if (ns > 2 && !io_ok(st[ns - 1])) {
+    unsigned after = 0;
+    for (int k = ns - 1; k >= 1; --k) {
+      if (!(st[k].mask & after) && io_ok(st[k])) {
+        std::rotate(st.begin() + k, st.begin() + k + 1, st.end());
+        break;
+      }
+      after |= st[k].mask;
+    }
+  }
   std::vector<TileOp> out;
   out.reserve(nops + st.size());
   for (const Stage &s : st) {
