@@ -850,6 +850,7 @@ __device__ __forceinline__ void stage_apply(T (&v)[NG][4][4], int a, const TileO
 template <typename T, int M, bool DENSE, bool GEN2Q, int NT, bool SWZ, bool VEC = true>
 __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, int nops, const T *__restrict__ prm) {
   constexpr int E = 1 << (2 * M);
+  constexpr int GPT = (E / 16 + NT - 1) / NT;  // groups of 16 per thread
   using Q = Pack<T, 4>;
   const int tid = threadIdx.x;
   for (int i = 0; i < nops; ++i) {
@@ -871,22 +872,42 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
       kb[u] = lds_swz<SWZ>(u << (2 * jb));
     }
     if (ja == 0) {
-      for (int g = tid; g < E / 16; g += NT) {
-        const int b = SWZ ? lds_swz<true>(lane_deposit(g, op.lanes, M - 2)) : ins_zero(ins_zero(g, 0), jb);
-        T v[1][4][4];
+      // VEC kernels with 32+ FP32 coefficients per thread (7-digit tiles of
+      // 512 threads: 2 groups) load all of a thread's groups before the
+      // first operation; the others take one group at a time (their
+      // register budget)
+      constexpr int GB = (VEC && sizeof(T) == 4 && E / NT >= 32) ? GPT : 1;
+      for (int g0 = tid; g0 < E / 16; g0 += GB * NT) {
+        int bq[GB];
+        T v[GB][4][4];
 #pragma unroll
-        for (int y = 0; y < 4; ++y) {
-          const Q qd = *reinterpret_cast<const Q *>(&s[b ^ kb[y]]);
+        for (int q = 0; q < GB; ++q) {
+          const int g = g0 + q * NT;
+          bq[q] = SWZ ? lds_swz<true>(lane_deposit(g, op.lanes, M - 2)) : ins_zero(ins_zero(g, 0), jb);
+          if (g < E / 16) {
 #pragma unroll
-          for (int x = 0; x < 4; ++x) v[0][x][y] = qd.v[x];
+            for (int y = 0; y < 4; ++y) {
+              const Q qd = *reinterpret_cast<const Q *>(&s[bq[q] ^ kb[y]]);
+#pragma unroll
+              for (int x = 0; x < 4; ++x) v[q][x][y] = qd.v[x];
+            }
+          }
         }
-        stage_apply<T, GEN2Q, 1>(v, 0, mem, nm, prm);
+        stage_apply<T, GEN2Q, GB>(v, 0, mem, nm, prm);
 #pragma unroll
-        for (int y = 0; y < 4; ++y) {
-          Q qd;
+        for (int q = 0; q < GB; ++q) {
+          if (g0 + q * NT >= E / 16) continue;
+          // the (I, I) coefficient never changes; left visible, the compiler
+          // drops it from the first quad's store and splits that store into
+          // 4-byte writes that hit each bank 4 times
+          asm volatile("" : "+v"(v[q][0][0]));
 #pragma unroll
-          for (int x = 0; x < 4; ++x) qd.v[x] = v[0][x][y];
-          *reinterpret_cast<Q *>(&s[b ^ kb[y]]) = qd;
+          for (int y = 0; y < 4; ++y) {
+            Q qd;
+#pragma unroll
+            for (int x = 0; x < 4; ++x) qd.v[x] = v[q][x][y];
+            *reinterpret_cast<Q *>(&s[bq[q] ^ kb[y]]) = qd;
+          }
         }
       }
     } else if (VEC && sizeof(T) == 4 && E / NT >= 32) {
