@@ -292,7 +292,10 @@ int dm_partition(int n, int ninst, const int32_t *kind, const int32_t *q0, const
  * (the candidate with the most gates), DM_PLAN_LEARNED (candidates scored by
  * their predicted net saving under the cost model mlp_params, with the
  * lookahead widths lookahead[0..nlookahead)); tiles != 0 adds the tile-wide
- * candidates of the learned planner.  Output: blocks in execution order,
+ * candidates of the learned planner.  precision (DM_FP64 / DM_FP32) is the
+ * state's: it selects the cost model's precision and the tile sizes (FP32
+ * runs 3 qubits above the three lowest positions in 7-digit tiles, FP64 in
+ * 6-digit ones).  Output: blocks in execution order,
  * out_members[out_block_start[b] ..] (gate indices) and
  * out_qubits[out_qubit_start[b] ..] (qubits in order of first use); out
  * arrays hold ngates, ngates + 1, 2 * ngates and ngates + 1 entries. */

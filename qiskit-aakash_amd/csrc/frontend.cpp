@@ -198,9 +198,10 @@ std::vector<int> near_qubits(const std::vector<Gate> &g, const std::vector<char>
 }
 
 // Qubits of the tile of a sequential pass over T (outside the low positions):
-// T, the low positions, filled up to 6 digits (|T| <= 3) or 7 (|T| = 4).
-uint32_t tile_mask(int n, const std::vector<int> &T) {
-  const int m = std::min(n, T.size() <= 3 ? 6 : 7);
+// T, the low positions, filled up to 6 digits (|T| <= 2, and |T| = 3 in
+// FP64) or 7 (|T| = 4, and |T| = 3 in FP32: runs of 4^4 coefficients).
+uint32_t tile_mask(int n, const std::vector<int> &T, bool fp32) {
+  const int m = std::min(n, T.size() <= 2 || (T.size() == 3 && !fp32) ? 6 : 7);
   uint32_t S = mask_of(T) | low_mask(n);
   for (int p = 0; popc(S) < m; ++p) S |= 1u << (n - 1 - p);
   return S;
@@ -242,7 +243,7 @@ void combinations(const std::vector<int> &pool, int k, F &&emit) {
 // positions (greedy growth counting only such qubits, and the seed's high
 // qubits with nearby ones), every gate the whole tile of T admits.
 std::vector<Block> candidate_blocks(const std::vector<Gate> &g, const std::vector<char> &done, int start, int kmax,
-                                    int n, bool tiles) {
+                                    int n, bool tiles, bool fp32) {
   std::vector<Block> out;
   std::set<std::vector<int>> keys;
   auto add = [&](const std::vector<int> &members) {
@@ -296,7 +297,7 @@ std::vector<Block> candidate_blocks(const std::vector<Gate> &g, const std::vecto
     });
   }
   for (const std::vector<int> &T : sets) {
-    absorb(g, done, start, tile_mask(n, T), &mem);
+    absorb(g, done, start, tile_mask(n, T, fp32), &mem);
     add(mem);
   }
   return out;
@@ -314,9 +315,12 @@ int run_digits(int n, uint32_t positions, int m) {
   return w;
 }
 
-int tile_for(int n, uint32_t positions, int k) {
-  if (n >= 7 && (k > 6 || (run_digits(n, positions, 6) < 3 && run_digits(n, positions, 7) > run_digits(n, positions, 6))))
-    return 7;
+// Tile digits of a sequential pass over `positions` (k qubits): 7 where 6
+// would leave runs of 4^2 coefficients, and in FP32 where 7 turns runs of
+// 4^3 into runs of 4^4 (measured faster on MI355X); else 6.
+int tile_for(int n, uint32_t positions, int k, bool fp32) {
+  const int w6 = run_digits(n, positions, 6), w7 = run_digits(n, positions, 7);
+  if (n >= 7 && (k > 6 || (w6 < 3 && w7 > w6) || (fp32 && w6 == 3 && w7 == 4))) return 7;
   return 6;
 }
 
@@ -364,7 +368,7 @@ struct CostModel {
     x[5] = (float)(log2(1.0 + st.n1) / 4.0);
     x[6] = (float)(log2(1.0 + st.n2) / 4.0);
     int w;
-    if (kind == K_SEQ) w = run_digits(n, pm, tile_for(n, pm, (int)qubits.size()));
+    if (kind == K_SEQ) w = run_digits(n, pm, tile_for(n, pm, (int)qubits.size(), !fp64));
     else if (kind == K_DENSE) w = run_digits(n, pm, 6);
     else w = pos[0] > 0 ? 1 : 0;
     x[7] = (float)(w / 6.0);
@@ -453,7 +457,7 @@ std::vector<double> scores(PlanCtx &c, const std::vector<Block> &cands, double p
 // score alone
 double best_after(PlanCtx &c, const std::vector<char> &done, int first, int depth, double per_gate) {
   if (first >= (int)c.g.size()) return 0.0;
-  const std::vector<Block> cands = candidate_blocks(c.g, done, first, c.km, c.n, c.tiles);
+  const std::vector<Block> cands = candidate_blocks(c.g, done, first, c.km, c.n, c.tiles, !c.cost->fp64);
   const std::vector<double> s = scores(c, cands, per_gate);
   if (depth >= (int)c.widths.size()) return *std::max_element(s.begin(), s.end());
   double best = 0.0;
@@ -686,7 +690,7 @@ int dm_plan_fusion(int n, int ngates, const int32_t *gate_qubits, int method, in
       grow(g, done, first, km, n, 0, &qs, &mem);
       pick = Block{qs, mem};
     } else {
-      std::vector<Block> cands = candidate_blocks(g, done, first, km, n, ctx.tiles);
+      std::vector<Block> cands = candidate_blocks(g, done, first, km, n, ctx.tiles, precision != DM_FP64);
       if (method == DM_PLAN_SEARCH) {
         size_t best = 0;
         for (size_t i = 1; i < cands.size(); ++i)

@@ -38,9 +38,9 @@ PARAMS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'costmode
 KINDS = ('plain1q', 'plaincx', 'seq', 'dense')
 
 
-def tile_run_digits(n, positions):
+def tile_run_digits(n, positions, precision='fp32'):
     """w of the tile a sequential pass over these positions uses."""
-    return fusion.run_digits(n, positions, fusion.tile_for(n, [n - 1 - p for p in positions]))
+    return fusion.run_digits(n, positions, fusion.tile_for(n, [n - 1 - p for p in positions], precision))
 
 
 def gate_stats(n, gates):
@@ -62,7 +62,7 @@ def features(kind, n, qubits, n1, n2, precision, n2low=0, d2=0):
     x[5] = math.log2(1 + n1) / 4.0
     x[6] = math.log2(1 + n2) / 4.0
     if kind == 'seq':
-        w = tile_run_digits(n, pos)
+        w = tile_run_digits(n, pos, precision)
     elif kind == 'dense':
         w = fusion.run_digits(n, pos, 6)
     else:
@@ -190,7 +190,7 @@ def default_model():
 # data collection and training (run on the GPU)
 # ---------------------------------------------------------------------------
 
-def candidate_passes(seeds, n, gates_per_circuit=200, kmax=4):
+def candidate_passes(seeds, n, gates_per_circuit=200, kmax=4, precision='fp32'):
     """Distinct pass specs (kind, qubits, gate list) drawn from the candidate
     blocks the planners consider (fusion.candidate_blocks) on seeded random
     circuits; each circuit advances by a randomly chosen candidate."""
@@ -205,7 +205,8 @@ def candidate_passes(seeds, n, gates_per_circuit=200, kmax=4):
                 first += 1
             if first >= len(noisy):
                 break
-            options = fusion.candidate_blocks(noisy, done, first, kmax, n, n >= fusion.MIN_TILE_QUBITS)
+            options = fusion.candidate_blocks(noisy, done, first, kmax, n, n >= fusion.MIN_TILE_QUBITS,
+                                              precision=precision)
             for qs, mem in options:
                 gs = [noisy[i] for i in mem]
                 modes = ('seq', 'dense') if 2 <= len(qs) <= 4 and len(mem) > 1 else ('seq',)
@@ -226,7 +227,7 @@ def time_pass(st, kind, qs, gates, reps=3):
     mode = 'dense' if kind == 'dense' else 'seq'
     idx = list(range(len(gates)))
     for _ in range(reps):
-        fusion.add_block(prog, gates, qs, idx, mode)
+        fusion.add_block(prog, gates, qs, idx, mode, precision='fp64' if st.dtype == native.FP64 else 'fp32')
     prog.run(st)  # warm-up (and upload of the pass's matrices)
     st.synchronize()
     st.profile(True)

@@ -214,12 +214,13 @@ def low_qubits(n):
     return frozenset(range(max(0, n - LOW_DIGITS), n))
 
 
-def tile_qubits(n, T):
+def tile_qubits(n, T, precision='fp32'):
     """Qubits of the tile a sequential pass over the qubits ``T`` (outside the
     low positions) runs in: T, the low positions, and the next lowest
-    positions up to 6 digits (|T| <= 3) or 7 digits (|T| = 4).  Its runs are
-    at least 4^3 coefficients long."""
-    m = min(n, 6 if len(T) <= 3 else 7)
+    positions up to 6 digits (|T| <= 2, and |T| = 3 in FP64) or 7 digits
+    (|T| = 4, and |T| = 3 in FP32, whose runs are then 4^4 coefficients).
+    Its runs are at least 4^3 coefficients long."""
+    m = min(n, 6 if len(T) <= 2 or (len(T) == 3 and precision == 'fp64') else 7)
     S = set(T) | low_qubits(n)
     p = 0
     while len(S) < m:
@@ -238,7 +239,7 @@ def used_qubits(gates, members):
     return qs
 
 
-def candidate_blocks(gates, done, start, kmax, n, tiles=False, masks=None):
+def candidate_blocks(gates, done, start, kmax, n, tiles=False, masks=None, precision='fp32'):
     """Distinct candidate blocks (qubits, members) seeded at ``start``.
 
     The greedy growth for every k <= kmax, and every seed-containing set of
@@ -249,7 +250,7 @@ def candidate_blocks(gates, done, start, kmax, n, tiles=False, masks=None):
     for every k, and the seed's qubits with nearby ones), every gate
     ``_absorb`` admits into T's whole tile (``tile_qubits``).  A block's
     qubits are those its members use; the single seed gate is always a
-    candidate."""
+    candidate.  ``precision`` selects the tile sizes (``tile_qubits``)."""
     if masks is None:
         masks = qubit_masks(gates)
     seed = list(gates[start].qubits)
@@ -280,7 +281,7 @@ def candidate_blocks(gates, done, start, kmax, n, tiles=False, masks=None):
         for X in itertools.combinations(near, extra):
             sets.add(frozenset(hi) | frozenset(X))
     for T in sorted(sets, key=sorted):
-        add(_absorb(gates, done, start, tile_qubits(n, T), masks))
+        add(_absorb(gates, done, start, tile_qubits(n, T, precision), masks))
     return list(cands.values())
 
 
@@ -318,7 +319,7 @@ def _lookahead_pick(gates, done, first, cands, score, per_gate, widths, ctx):
         if f2 >= len(gates):
             return 0.0
         if key not in seen:
-            c2 = candidate_blocks(gates, d2, f2, km, n, tiles, masks)
+            c2 = candidate_blocks(gates, d2, f2, km, n, tiles, masks, cost.precision)
             t2 = cost.block_times(n, [(q2, [gates[j] for j in m2]) for q2, m2 in c2])
             seen[key] = (f2, c2, [len(m2) * per_gate - TIME_WEIGHT * tt for (_, m2), tt in zip(c2, t2)])
         _, c2, s2 = seen[key]
@@ -335,18 +336,20 @@ def _lookahead_pick(gates, done, first, cands, score, per_gate, widths, ctx):
     return pick, ((nxt[0], nxt[1]) if nxt is not None else None)
 
 
-def plan(gates, n, kmax=4, method='greedy', cost=None, tiles=True, lookahead=LOOKAHEAD):
+def plan(gates, n, kmax=4, method='greedy', cost=None, tiles=True, lookahead=LOOKAHEAD, precision=None):
     """Blocks as (qubits, [gate indices]) in execution order, from the native
     planner (dm_plan_fusion, the same rules as ``plan_py``).  With ``tiles``
     (blocks run as sequential tile passes of the default size) the learned
     planner's candidates include the tile-wide blocks (``candidate_blocks``);
-    ``lookahead`` gives its search widths (LOOKAHEAD), () none."""
+    ``lookahead`` gives its search widths (LOOKAHEAD), () none.  The state's
+    ``precision`` defaults to the cost model's, else FP32."""
     if method not in native.PLAN_METHODS:
         raise ValueError('unknown fusion method %r' % method)
+    prec = precision or (cost.precision if cost is not None else 'fp32')
     gq = [(g.qubits[0], g.qubits[1] if len(g.qubits) == 2 else -1) for g in gates]
     if method == 'learned':
-        return native.plan_fusion(n, gq, method, kmax, tiles, tuple(lookahead), cost.params, cost.precision)
-    return native.plan_fusion(n, gq, method, kmax, tiles)
+        return native.plan_fusion(n, gq, method, kmax, tiles, tuple(lookahead), cost.params, prec)
+    return native.plan_fusion(n, gq, method, kmax, tiles, precision=prec)
 
 
 def plan_py(gates, n, kmax=4, method='greedy', cost=None, tiles=True, lookahead=LOOKAHEAD):
@@ -373,7 +376,8 @@ def plan_py(gates, n, kmax=4, method='greedy', cost=None, tiles=True, lookahead=
             if ahead is not None and ahead[0] == first:
                 cands = ahead[1]
             else:
-                cands = candidate_blocks(gates, done, first, km, n, tiles, masks)
+                cands = candidate_blocks(gates, done, first, km, n, tiles, masks,
+                                         cost.precision if cost is not None else 'fp32')
             ahead = None
             if method == 'learned':
                 # predicted net saving: the gates a block takes, valued at the
@@ -446,19 +450,21 @@ def run_digits(n, positions, m):
     return w
 
 
-def tile_for(n, qs):
+def tile_for(n, qs, precision='fp32'):
     """Tile size of a sequential pass over block ``qs``: 6 digits (32 KB of
     LDS in FP64) unless that leaves runs of only 4^2 coefficients, which
-    stream at about two thirds of the rate of longer runs; then 7 digits
-    (512 threads, 128 KB in FP64) when the register allows it.  A block of 7
-    qubits takes 7 digits."""
+    stream at about two thirds of the rate of longer runs, or (FP32) 7 digits
+    turn runs of 4^3 into runs of 4^4 (measured faster on MI355X); then 7
+    digits (512 threads, 64 / 128 KB) when the register allows it.  A block
+    of 7 qubits takes 7 digits."""
     pos = [n - 1 - q for q in qs]
-    if n >= 7 and (len(qs) > 6 or (run_digits(n, pos, 6) < 3 and run_digits(n, pos, 7) > run_digits(n, pos, 6))):
+    w6, w7 = run_digits(n, pos, 6), run_digits(n, pos, 7)
+    if n >= 7 and (len(qs) > 6 or (w6 < 3 and w7 > w6) or (precision != 'fp64' and w6 == 3 and w7 == 4)):
         return 7
     return 6
 
 
-def add_block(prog, gates, qs, mem, mode='seq', tile=None):
+def add_block(prog, gates, qs, mem, mode='seq', tile=None, precision='fp32'):
     """Append one block to a Program.
 
     A single gate runs on its plain kernel.  Otherwise, with n >= 6:
@@ -479,7 +485,7 @@ def add_block(prog, gates, qs, mem, mode='seq', tile=None):
     sub = [gates[i] for i in mem]
     n = prog.n
     if tile is None:
-        tile = tile_for(n, qs) if n >= 6 else 6
+        tile = tile_for(n, qs, precision) if n >= 6 else 6
     if len(qs) == 1:
         prog.one_qubit(qs[0], block_ptm(sub, qs))
     elif (mode == 'seq' or len(qs) > 4) and n >= tile:
@@ -504,18 +510,22 @@ def plain_program(gates, n):
     return prog
 
 
-def fused_program(gates, n, kmax=4, method='greedy', cost=None, mode='seq', tile=None, lookahead=LOOKAHEAD):
+def fused_program(gates, n, kmax=4, method='greedy', cost=None, mode='seq', tile=None, lookahead=LOOKAHEAD,
+                  precision=None):
     """Program of fused passes; with a cost model each block also gets the
     cheaper of the sequential and dense forms.  ``tile`` fixes the tile size
-    of sequential passes (default: ``tile_for``); blocks take the gates of
-    their whole tile only with the default size in mode 'seq'."""
-    blocks = plan(gates, n, kmax, method, cost, tiles=(mode == 'seq' and tile is None), lookahead=lookahead)
+    of sequential passes (default: ``tile_for`` for the state's
+    ``precision``, by default the cost model's, else FP32); blocks take the
+    gates of their whole tile only with the default size in mode 'seq'."""
+    prec = precision or (cost.precision if cost is not None else 'fp32')
+    blocks = plan(gates, n, kmax, method, cost, tiles=(mode == 'seq' and tile is None), lookahead=lookahead,
+                  precision=prec)
     prog = Program(n)
     for qs, mem in blocks:
         m = mode
         if cost is not None and len(mem) > 1:
             m = cost.best_mode(n, qs, [gates[i] for i in mem])
-        add_block(prog, gates, qs, mem, m, tile)
+        add_block(prog, gates, qs, mem, m, tile, prec)
     return prog, blocks
 
 
@@ -556,6 +566,7 @@ def plan_levels(levels, count, n, backend):
             clock += 1
         # below 15 qubits a pass takes about a millisecond and the lookahead's
         # host time (about 2x the plain choice) costs more than it saves
-        prog, _ = fused_program(gates, n, KMAX[method], method, cost, lookahead=LOOKAHEAD if n >= 15 else ())
+        prog, _ = fused_program(gates, n, KMAX[method], method, cost, lookahead=LOOKAHEAD if n >= 15 else (),
+                                precision=backend._precision)
         out[start] = (clock, prog.run)
     return out

@@ -221,17 +221,22 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_fused_tile_choice():
-    """Sequential passes take 7-digit tiles only where 6 digits would leave
-    contiguous HBM runs of 4^2 coefficients."""
+    """Sequential passes take 7-digit tiles where 6 digits would leave
+    contiguous HBM runs of 4^2 coefficients, and in FP32 where 7 digits turn
+    runs of 4^3 into 4^4; FP64 keeps 6 digits there."""
     from dmsim import fusion
     n = 16
     assert fusion.run_digits(n, [15, 14, 13, 12], 6) == 2
     assert fusion.run_digits(n, [15, 14, 13, 12], 7) == 3
     assert fusion.run_digits(n, [0, 7], 6) == 5
     assert fusion.tile_for(n, [0, 1, 2, 3]) == 7      # positions 12..15
-    assert fusion.tile_for(n, [0, 1, 2]) == 6         # m=6 already gives w=3
+    assert fusion.tile_for(n, [0, 1, 2]) == 7         # FP32: w 3 -> 4
+    assert fusion.tile_for(n, [0, 1, 2], 'fp64') == 6  # FP64: m=6 already gives w=3
+    assert fusion.tile_for(n, [0, 1, 12]) == 6        # position 3 in the block: m=6 gives w=4
     assert fusion.tile_for(n, [15, 14]) == 6          # block at the lowest positions
     assert fusion.tile_for(6, [0, 1, 2, 3]) == 6      # register too small for 7
+    assert fusion.tile_qubits(n, {0, 1, 2}) == {0, 1, 2, 12, 13, 14, 15}
+    assert fusion.tile_qubits(n, {0, 1, 2}, 'fp64') == {0, 1, 2, 13, 14, 15}
 
 
 def test_shipped_cost_model_file():
