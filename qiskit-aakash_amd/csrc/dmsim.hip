@@ -1052,11 +1052,13 @@ __global__ __launch_bounds__(NT) void k_tile_prog(T *__restrict__ a, TileGeom ge
 // first), or two digits >= 3 (lanes over digits 0-2 first; FP32 7-digit
 // tiles pair the groups that differ in the low bit of digit 0 into 8-byte
 // accesses).  A thread loads all its groups before the first operation.
+// NTM: nontemporal HBM accesses (one-tile passes over whole states; the
+// batch's small states keep them cached).
 __host__ __device__ __forceinline__ bool stage_io_ok(const TileOp &op) {
   return (op.d[0] == 0 && op.d[1] >= 3) || op.d[0] >= 3;
 }
 
-template <typename T, int M, bool GEN2Q, int NT, bool SWZ, bool VEC, int DIR>
+template <typename T, int M, bool GEN2Q, int NT, bool SWZ, bool VEC, int DIR, bool NTM = true>
 __device__ __forceinline__ void stage_io(T *__restrict__ a, uint64_t base, int w, int lomask, const TileGeom &geo,
                                          T *s, const TileOp *__restrict__ ops, int at, const T *__restrict__ prm) {
   constexpr int E = 1 << (2 * M);
@@ -1095,7 +1097,8 @@ __device__ __forceinline__ void stage_io(T *__restrict__ a, uint64_t base, int w
             p = *reinterpret_cast<const P2 *>(&s[lb ^ ka[x] ^ kb[y]]);
           } else {
             using V2 = T __attribute__((ext_vector_type(2)));
-            const V2 q = __builtin_nontemporal_load(reinterpret_cast<const V2 *>(hp + ha[x] + hb[y]));
+            const V2 *qp = reinterpret_cast<const V2 *>(hp + ha[x] + hb[y]);
+            const V2 q = NTM ? __builtin_nontemporal_load(qp) : *qp;
             p.v[0] = q[0];
             p.v[1] = q[1];
           }
@@ -1117,7 +1120,8 @@ __device__ __forceinline__ void stage_io(T *__restrict__ a, uint64_t base, int w
             V2 q;
             q[0] = v[0][x][y];
             q[1] = v[1][x][y];
-            __builtin_nontemporal_store(q, reinterpret_cast<V2 *>(hp + ha[x] + hb[y]));
+            if (NTM) __builtin_nontemporal_store(q, reinterpret_cast<V2 *>(hp + ha[x] + hb[y]));
+            else *reinterpret_cast<V2 *>(hp + ha[x] + hb[y]) = q;
           }
         }
     }
@@ -1135,7 +1139,7 @@ __device__ __forceinline__ void stage_io(T *__restrict__ a, uint64_t base, int w
     if (ja == 0) {
 #pragma unroll
       for (int y = 0; y < 4; ++y) {
-        const Q qd = DIR == 1 ? lds_get_quad(s, lb[q] ^ kb[y]) : ldq<T, true>(hp[q] + hb[y]);
+        const Q qd = DIR == 1 ? lds_get_quad(s, lb[q] ^ kb[y]) : ldq<T, NTM>(hp[q] + hb[y]);
 #pragma unroll
         for (int x = 0; x < 4; ++x) v[q][x][y] = qd.v[x];
       }
@@ -1144,7 +1148,8 @@ __device__ __forceinline__ void stage_io(T *__restrict__ a, uint64_t base, int w
       for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int y = 0; y < 4; ++y)
-          v[q][x][y] = DIR == 1 ? s[lb[q] ^ ka[x] ^ kb[y]] : __builtin_nontemporal_load(hp[q] + ha[x] + hb[y]);
+          v[q][x][y] = DIR == 1 ? s[lb[q] ^ ka[x] ^ kb[y]]
+                                : NTM ? __builtin_nontemporal_load(hp[q] + ha[x] + hb[y]) : hp[q][ha[x] + hb[y]];
     }
   }
   stage_apply<T, GEN2Q, GPT>(v, ja, mem, nm, prm);
@@ -1157,7 +1162,7 @@ __device__ __forceinline__ void stage_io(T *__restrict__ a, uint64_t base, int w
 #pragma unroll
         for (int x = 0; x < 4; ++x) qd.v[x] = v[q][x][y];
         if (DIR == 0) lds_put_quad(s, lb[q] ^ kb[y], qd);
-        else stq<T, true>(hp[q] + hb[y], qd);
+        else stq<T, NTM>(hp[q] + hb[y], qd);
       }
     } else {
 #pragma unroll
@@ -1165,7 +1170,8 @@ __device__ __forceinline__ void stage_io(T *__restrict__ a, uint64_t base, int w
 #pragma unroll
         for (int y = 0; y < 4; ++y) {
           if (DIR == 0) s[lb[q] ^ ka[x] ^ kb[y]] = v[q][x][y];
-          else __builtin_nontemporal_store(v[q][x][y], hp[q] + ha[x] + hb[y]);
+          else if (NTM) __builtin_nontemporal_store(v[q][x][y], hp[q] + ha[x] + hb[y]);
+          else hp[q][ha[x] + hb[y]] = v[q][x][y];
         }
     }
   }
@@ -1617,6 +1623,8 @@ struct BatchPass {
   int32_t circ;       // HBM-tier circuit index (clock stamps)
   int32_t n;          // register size
   int32_t nops;       // BP_TILE: tile operations
+  int32_t io;         // BP_TILE: stage_io bits (first / last stage reads / writes HBM)
+  int32_t last_at;    // BP_TILE: header index of the last operation
   int64_t top_off;    // BP_TILE: first tile operation
   int64_t state_off;  // first coefficient of the circuit's state in the arena
   int64_t out_off;    // BP_PROBS: first probability
@@ -1645,24 +1653,42 @@ __device__ __forceinline__ void queue_tiles(T *s, T *a, const BatchPass &bp, uin
   const int tid = threadIdx.x;
   const int w = bp.geo.w;
   const int lomask = (1 << (2 * w)) - 1;
+  const TileOp *ops = tops + bp.top_off;
   for (uint32_t tile = t0; tile < t0 + nt; ++tile) {
     uint64_t base = 0, r = tile;
     for (int j = 0; j < bp.geo.nr; ++j) {
       base |= (r & 3ull) << (2 * bp.geo.rpos[j]);
       r >>= 2;
     }
-    Q buf[QPT];
+    // the first and last stage read and write HBM themselves where
+    // stage_io allows (bp.io), as in the one-tile kernel
+    if ((bp.io & 3) == 3 && bp.last_at == 0) {
+      stage_io<T, M, true, kQueueThreads, SWZ, false, 2, false>(a, base, w, lomask, bp.geo, s, ops, 0, prm);
+      continue;
+    }
+    int i0 = 0;
+    if (bp.io & 1) {
+      stage_io<T, M, true, kQueueThreads, SWZ, false, 0, false>(a, base, w, lomask, bp.geo, s, ops, 0, prm);
+      i0 = 1 + ops[0].k;
+    } else {
+      Q buf[QPT];
 #pragma unroll
-    for (int i = 0; i < QPT; ++i)
-      buf[i] = ldp<Q>(a + base + tile_offset<T, M>(4 * (tid + i * kQueueThreads), w, lomask, bp.geo));
+      for (int i = 0; i < QPT; ++i)
+        buf[i] = ldp<Q>(a + base + tile_offset<T, M>(4 * (tid + i * kQueueThreads), w, lomask, bp.geo));
 #pragma unroll
-    for (int i = 0; i < QPT; ++i) lds_put_quad(s, lds_swz<SWZ>(4 * (tid + i * kQueueThreads)), buf[i]);
+      for (int i = 0; i < QPT; ++i) lds_put_quad(s, lds_swz<SWZ>(4 * (tid + i * kQueueThreads)), buf[i]);
+    }
     __syncthreads();
-    tile_ops<T, M, false, true, kQueueThreads, SWZ>(s, tops + bp.top_off, bp.nops, prm);
+    const int i1 = (bp.io & 2) ? bp.last_at : bp.nops;
+    tile_ops<T, M, false, true, kQueueThreads, SWZ>(s, ops + i0, i1 - i0, prm);
+    if (bp.io & 2) {
+      stage_io<T, M, true, kQueueThreads, SWZ, false, 1, false>(a, base, w, lomask, bp.geo, s, ops, bp.last_at, prm);
+    } else {
 #pragma unroll
-    for (int i = 0; i < QPT; ++i) {
-      const int c = tid + i * kQueueThreads;
-      stp(a + base + tile_offset<T, M>(4 * c, w, lomask, bp.geo), lds_get_quad(s, lds_swz<SWZ>(4 * c)));
+      for (int i = 0; i < QPT; ++i) {
+        const int c = tid + i * kQueueThreads;
+        stp(a + base + tile_offset<T, M>(4 * c, w, lomask, bp.geo), lds_get_quad(s, lds_swz<SWZ>(4 * c)));
+      }
     }
     __syncthreads();  // LDS reads of this tile before the next tile's writes
   }
@@ -2681,6 +2707,19 @@ int stream_resident_per_cu() {
   return cached;
 }
 
+// stage_io bits of a pass (bit 0: its first operation is a stage that can
+// read HBM itself, bit 1: its last one can write HBM; stage_io_ok) and the
+// header index of its last operation.
+int stage_io_flags(const FusedPass &fp, int *last) {
+  const int nops = (int)fp.ops.size();
+  int last_stage = 0, io = 0;
+  for (int i = 0; i < nops; i += fp.ops[i].kind == TOP_STAGE ? 1 + fp.ops[i].k : 1) last_stage = i;
+  if (nops > 0 && fp.ops[0].kind == TOP_STAGE && stage_io_ok(fp.ops[0])) io |= 1;
+  if (nops > 0 && fp.ops[last_stage].kind == TOP_STAGE && stage_io_ok(fp.ops[last_stage])) io |= 2;
+  *last = last_stage;
+  return io;
+}
+
 template <typename T>
 int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T *d_vals) {
   ProfScope ps(h, 9);
@@ -2692,12 +2731,9 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
   }
   T *a = static_cast<T *>(h->d_a);
   // the one-tile kernel fuses the first and the last stage with the HBM
-  // copy where their group order suits both HBM and LDS (stage_io_ok): io
-  // bits 0 and 1, last_stage the header index of the last operation
-  int last_stage = 0, io = 0;
-  for (int i = 0; i < nops; i += fp.ops[i].kind == TOP_STAGE ? 1 + fp.ops[i].k : 1) last_stage = i;
-  if (nops > 0 && fp.ops[0].kind == TOP_STAGE && stage_io_ok(fp.ops[0])) io |= 1;
-  if (nops > 0 && fp.ops[last_stage].kind == TOP_STAGE && stage_io_ok(fp.ops[last_stage])) io |= 2;
+  // copy where their group order suits both HBM and LDS
+  int last_stage = 0;
+  const int io = stage_io_flags(fp, &last_stage);
   // one variant per feature set keeps the register footprint of the common
   // 1q + CX passes small; 7-qubit tiles (64 / 128 KB of LDS) use 512 threads
   // Grid shape, from 16-qubit sweeps on MI355X:
@@ -3314,6 +3350,9 @@ static int batch_create_t(dm_batch *b, const int64_t *op_start, const dm_op *ops
         tp.kind = BP_TILE;
         tp.geo = fp.geo;
         tp.nops = (int32_t)fp.ops.size();
+        int last_at = 0;
+        tp.io = stage_io_flags(fp, &last_at);
+        tp.last_at = last_at;
         tp.top_off = (int64_t)qtops.size();
         const size_t vbase = qvals.size();
         for (TileOp t : fp.ops) {
