@@ -298,21 +298,24 @@ def tile_ops_mode(args):
     for rocprofv3 LDS counter passes): for 6- and 7-digit tiles over the
     highest qubits, one launch each of a pass with no operations, then with 8
     copies of one operation: a 1q PTM on a high digit, on tile digit 0 and on
-    tile digit 1, a CX on two high digits, on digits 0 and 1 and on digits
-    2 and high (the last three swizzled where they touch digits 0-2).
+    tile digit 1, a CX on two high digits, on digits 0 and 1, on digit 2 and
+    the highest digit, on digits 1 and 3 and on digits 2 and 4 (swizzled
+    where they touch digits 0-2).
     Returns the milliseconds of each pass (HIP events)."""
     from dmsim.program import Program
     n = args.qubits
     R = ptm.rows3(ptm.depolarizing1(1e-3) @ ptm.gate_ptm('u3', [0.3, 0.2, 0.1]))
     S = ptm.depolarizing_pair_scale(1e-3)
-    lo = [n - 1, n - 2, n - 3]   # qubits at positions 0, 1, 2
-    cases = [('empty', []), ('1q high', [(native.SUB_1Q, [0], R)]), ('1q d0', [(native.SUB_1Q, [lo[0]], R)]),
-             ('1q d1', [(native.SUB_1Q, [lo[1]], R)]), ('cx high', [(native.SUB_CX, [0, 1], S)]),
-             ('cx d0 d1', [(native.SUB_CX, [lo[0], lo[1]], S)]), ('cx d2 high', [(native.SUB_CX, [lo[2], 0], S)])]
+    lo = [n - 1, n - 2, n - 3]   # qubits at positions 0, 1, 2 (tile digits 0, 1, 2)
     st = native.DeviceState(n, 'fp32')
     zero_state(st, n)
     out = {'n': n, 'ms': {}}
     for m, qs in ((6, [0, 1, 2]), (7, [0, 1, 2, 3])):
+        hi = qs[::-1]   # tile digits 3, 4, ...
+        cases = [('empty', []), ('1q high', [(native.SUB_1Q, [0], R)]), ('1q d0', [(native.SUB_1Q, [lo[0]], R)]),
+                 ('1q d1', [(native.SUB_1Q, [lo[1]], R)]), ('cx high', [(native.SUB_CX, [0, 1], S)]),
+                 ('cx d0 d1', [(native.SUB_CX, [lo[0], lo[1]], S)]), ('cx d2 high', [(native.SUB_CX, [lo[2], 0], S)]),
+                 ('cx d1 d3', [(native.SUB_CX, [lo[1], hi[0]], S)]), ('cx d2 d4', [(native.SUB_CX, [lo[2], hi[1]], S)])]
         for label, sub in cases:
             prog = Program(n)
             prog.fused(m, qs, sub * 8)

@@ -761,6 +761,15 @@ __device__ __forceinline__ uint64_t tile_offset(int t, int w, int lomask, const 
   return off;
 }
 
+// a * b + c with one rounding (v_fma_f32 / v_fma_f64).  The library is
+// compiled without floating-point contraction (its host planner must match
+// the Python planner bit for bit), so the register stages ask for it.
+template <typename T>
+__device__ __forceinline__ T fmad(T a, T b, T c) {
+  if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, c);
+  else return __builtin_fma(a, b, c);
+}
+
 // The member operations of a stage on NG groups of 16 coefficients in
 // registers: v[g][x][y], x the component index along stage digit a (the
 // lower one), y along the other digit.
@@ -783,7 +792,7 @@ __device__ __forceinline__ void stage_apply(T (&v)[NG][4][4], int a, const TileO
           for (int y = 0; y < 4; ++y) {
             const T x0 = v[g][0][y], x1 = v[g][1][y], x2 = v[g][2][y], x3 = v[g][3][y];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) v[g][c + 1][y] = r[c][0] * x0 + r[c][1] * x1 + r[c][2] * x2 + r[c][3] * x3;
+            for (int c = 0; c < 3; ++c) v[g][c + 1][y] = fmad(r[c][3], x3, fmad(r[c][2], x2, fmad(r[c][1], x1, r[c][0] * x0)));
           }
       } else {
 #pragma unroll
@@ -792,7 +801,7 @@ __device__ __forceinline__ void stage_apply(T (&v)[NG][4][4], int a, const TileO
           for (int x = 0; x < 4; ++x) {
             const T x0 = v[g][x][0], x1 = v[g][x][1], x2 = v[g][x][2], x3 = v[g][x][3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) v[g][x][c + 1] = r[c][0] * x0 + r[c][1] * x1 + r[c][2] * x2 + r[c][3] * x3;
+            for (int c = 0; c < 3; ++c) v[g][x][c + 1] = fmad(r[c][3], x3, fmad(r[c][2], x2, fmad(r[c][1], x1, r[c][0] * x0)));
           }
       }
     } else if (!GEN2Q || m.kind == TOP_CX) {
@@ -817,7 +826,7 @@ __device__ __forceinline__ void stage_apply(T (&v)[NG][4][4], int a, const TileO
         for (int r = 1; r < 16; ++r) {
           T acc = T(0);
 #pragma unroll
-          for (int c = 0; c < 16; ++c) acc += pv[16 * r + c] * (lo_first ? v[g][c >> 2][c & 3] : v[g][c & 3][c >> 2]);
+          for (int c = 0; c < 16; ++c) acc = fmad(pv[16 * r + c], lo_first ? v[g][c >> 2][c & 3] : v[g][c & 3][c >> 2], acc);
           w[r] = acc;
         }
 #pragma unroll
