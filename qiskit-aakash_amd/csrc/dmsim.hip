@@ -609,10 +609,13 @@ __global__ __launch_bounds__(kBlock) void k_block_mfma(T *__restrict__ a, const 
 //              order), applied in order to each group of 16 coefficients in
 //              registers: one LDS round trip for all of them.
 // The kernels run TOP_STAGE and TOP_DENSE records; the host turns every
-// other operation of a pass into a stage member (group_stages).
+// other operation of a pass into a stage member (group_stages), marks the
+// single-qubit members whose PTM is unital (first column (0, 0, 0): no term
+// in the identity component) as TOP_1QU, and gives every CX member its 16
+// scales (ones for a plain CX).
 // ---------------------------------------------------------------------------
 
-enum { TOP_1Q = 1, TOP_CX = 2, TOP_2Q = 3, TOP_DENSE = 4, TOP_STAGE = 6 };
+enum { TOP_1Q = 1, TOP_CX = 2, TOP_2Q = 3, TOP_DENSE = 4, TOP_STAGE = 6, TOP_1QU = 7 };
 
 struct TileOp {
   int32_t kind, k, d[4], param;
@@ -770,6 +773,41 @@ __device__ __forceinline__ T fmad(T a, T b, T c) {
   else return __builtin_fma(a, b, c);
 }
 
+// Single-qubit member on stage digit a (LO) or b: rows X, Y, Z of the PTM;
+// AFF adds the identity-component column (non-unital maps).
+template <typename T, int NG, bool LO, bool AFF>
+__device__ __forceinline__ void member_1q(T (&v)[NG][4][4], const T *__restrict__ pv) {
+  T r[3][4];
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) r[x][y] = pv[4 * x + y];
+#pragma unroll
+  for (int g = 0; g < NG; ++g)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+      const T x0 = LO ? v[g][0][y] : v[g][y][0], x1 = LO ? v[g][1][y] : v[g][y][1];
+      const T x2 = LO ? v[g][2][y] : v[g][y][2], x3 = LO ? v[g][3][y] : v[g][y][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T o = fmad(r[c][3], x3, fmad(r[c][2], x2, AFF ? fmad(r[c][1], x1, r[c][0] * x0) : r[c][1] * x1));
+        if (LO) v[g][c + 1][y] = o;
+        else v[g][y][c + 1] = o;
+      }
+    }
+}
+
+// CX member, control digit a (LO: pair (c, t) is v[c][t]) or b (v[t][c]),
+// with its 16 per-pair scales.
+template <typename T, int NG, bool LO>
+__device__ __forceinline__ void member_cx(T (&v)[NG][4][4], const T *__restrict__ pv) {
+  Scale16<T> sv;
+#pragma unroll
+  for (int u = 0; u < 16; ++u) sv.s[u] = pv[u];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) cx_map<T, T, LO, true>(v[g], sv);
+}
+
 // The member operations of a stage on NG groups of 16 coefficients in
 // registers: v[g][x][y], x the component index along stage digit a (the
 // lower one), y along the other digit.
@@ -779,46 +817,18 @@ __device__ __forceinline__ void stage_apply(T (&v)[NG][4][4], int a, const TileO
   for (int q = 0; q < nm; ++q) {
     const TileOp m = mem[q];
     const T *pv = prm + m.param;
-    if (m.kind == TOP_1Q) {
-      T r[3][4];
-#pragma unroll
-      for (int x = 0; x < 3; ++x)
-#pragma unroll
-        for (int y = 0; y < 4; ++y) r[x][y] = pv[4 * x + y];
-      if (m.d[0] == a) {
-#pragma unroll
-        for (int g = 0; g < NG; ++g)
-#pragma unroll
-          for (int y = 0; y < 4; ++y) {
-            const T x0 = v[g][0][y], x1 = v[g][1][y], x2 = v[g][2][y], x3 = v[g][3][y];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) v[g][c + 1][y] = fmad(r[c][3], x3, fmad(r[c][2], x2, fmad(r[c][1], x1, r[c][0] * x0)));
-          }
-      } else {
-#pragma unroll
-        for (int g = 0; g < NG; ++g)
-#pragma unroll
-          for (int x = 0; x < 4; ++x) {
-            const T x0 = v[g][x][0], x1 = v[g][x][1], x2 = v[g][x][2], x3 = v[g][x][3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) v[g][x][c + 1] = fmad(r[c][3], x3, fmad(r[c][2], x2, fmad(r[c][1], x1, r[c][0] * x0)));
-          }
-      }
+    const bool lo = m.d[0] == a;
+    if (m.kind == TOP_1QU) {
+      if (lo) member_1q<T, NG, true, false>(v, pv);
+      else member_1q<T, NG, false, false>(v, pv);
+    } else if (m.kind == TOP_1Q) {
+      if (lo) member_1q<T, NG, true, true>(v, pv);
+      else member_1q<T, NG, false, true>(v, pv);
     } else if (!GEN2Q || m.kind == TOP_CX) {
-      Scale16<T> sv;
-#pragma unroll
-      for (int u = 0; u < 16; ++u) sv.s[u] = m.k ? pv[u] : T(1);
-      // control digit a: pair (c, t) is v[c][t]; control digit b: v[t][c]
-      if (m.d[0] == a) {
-#pragma unroll
-        for (int g = 0; g < NG; ++g) cx_map<T, T, true, true>(v[g], sv);
-      } else {
-#pragma unroll
-        for (int g = 0; g < NG; ++g) cx_map<T, T, false, true>(v[g], sv);
-      }
+      if (lo) member_cx<T, NG, true>(v, pv);
+      else member_cx<T, NG, false>(v, pv);
     } else {
       // 16x16 PTM, pair index 4 i_d0 + i_d1 (first row the identity's)
-      const bool lo_first = m.d[0] == a;
 #pragma unroll
       for (int g = 0; g < NG; ++g) {
         T w[16];
@@ -826,12 +836,12 @@ __device__ __forceinline__ void stage_apply(T (&v)[NG][4][4], int a, const TileO
         for (int r = 1; r < 16; ++r) {
           T acc = T(0);
 #pragma unroll
-          for (int c = 0; c < 16; ++c) acc = fmad(pv[16 * r + c], lo_first ? v[g][c >> 2][c & 3] : v[g][c & 3][c >> 2], acc);
+          for (int c = 0; c < 16; ++c) acc = fmad(pv[16 * r + c], lo ? v[g][c >> 2][c & 3] : v[g][c & 3][c >> 2], acc);
           w[r] = acc;
         }
 #pragma unroll
         for (int r = 1; r < 16; ++r) {
-          if (lo_first) v[g][r >> 2][r & 3] = w[r];
+          if (lo) v[g][r >> 2][r & 3] = w[r];
           else v[g][r & 3][r >> 2] = w[r];
         }
       }
@@ -2519,6 +2529,16 @@ void group_stages(FusedPass *fp) {
     hd.d[1] = 31 - __builtin_clz(s.mask);
     out.push_back(hd);
     for (int i : s.mem) out.push_back(fp->ops[i]);
+  }
+  for (TileOp &op : out) {
+    if (op.kind == TOP_1Q) {
+      const double *r = &fp->vals[op.param];
+      if (r[0] == 0.0 && r[4] == 0.0 && r[8] == 0.0) op.kind = TOP_1QU;
+    } else if (op.kind == TOP_CX && op.k == 0) {
+      op.k = 1;
+      op.param = (int32_t)fp->vals.size();
+      fp->vals.insert(fp->vals.end(), 16, 1.0);
+    }
   }
   fp->ops.swap(out);
 }
