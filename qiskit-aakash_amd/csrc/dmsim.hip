@@ -696,6 +696,18 @@ __device__ __forceinline__ Pack<T, 4> lds_get_quad(const T *s, int lb) {
   return *reinterpret_cast<const Pack<T, 4> *>(&s[lb]);
 }
 
+// LDS accesses at a byte offset.  A stage addresses a coefficient as the
+// XOR of a per-thread offset and a uniform one (scalar registers); in bytes
+// that is one vector instruction per access instead of two.
+template <typename V, typename T>
+__device__ __forceinline__ V lds_ld(const T *s, uint32_t off) {
+  return *reinterpret_cast<const V *>(reinterpret_cast<const char *>(s) + off);
+}
+template <typename V, typename T>
+__device__ __forceinline__ void lds_st(T *s, uint32_t off, const V &x) {
+  *reinterpret_cast<V *>(reinterpret_cast<char *>(s) + off) = x;
+}
+
 __device__ __forceinline__ int ins_zero(int g, int j) {
   const int lo = g & ((1 << (2 * j)) - 1);
   return ((g >> (2 * j)) << (2 * j + 2)) | lo;
@@ -884,11 +896,11 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
     const TileOp *mem = ops + i + 1;
     const int nm = op.k;
     i += nm;
-    int ka[4], kb[4];
+    uint32_t ka[4], kb[4];  // byte offsets of the components along ja, jb
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      ka[u] = lds_swz<SWZ>(u << (2 * ja));
-      kb[u] = lds_swz<SWZ>(u << (2 * jb));
+      ka[u] = lds_swz<SWZ>(u << (2 * ja)) * (uint32_t)sizeof(T);
+      kb[u] = lds_swz<SWZ>(u << (2 * jb)) * (uint32_t)sizeof(T);
     }
     if (ja == 0) {
       // VEC kernels with 32+ FP32 coefficients per thread (7-digit tiles of
@@ -897,16 +909,16 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
       // register budget)
       constexpr int GB = (VEC && sizeof(T) == 4 && E / NT >= 32) ? GPT : 1;
       for (int g0 = tid; g0 < E / 16; g0 += GB * NT) {
-        int bq[GB];
+        uint32_t bq[GB];
         T v[GB][4][4];
 #pragma unroll
         for (int q = 0; q < GB; ++q) {
           const int g = g0 + q * NT;
-          bq[q] = SWZ ? lds_swz<true>(lane_deposit(g, op.lanes, M - 2)) : ins_zero(ins_zero(g, 0), jb);
+          bq[q] = (SWZ ? lds_swz<true>(lane_deposit(g, op.lanes, M - 2)) : ins_zero(ins_zero(g, 0), jb)) * (uint32_t)sizeof(T);
           if (g < E / 16) {
 #pragma unroll
             for (int y = 0; y < 4; ++y) {
-              const Q qd = *reinterpret_cast<const Q *>(&s[bq[q] ^ kb[y]]);
+              const Q qd = lds_ld<Q>(s, bq[q] ^ kb[y]);
 #pragma unroll
               for (int x = 0; x < 4; ++x) v[q][x][y] = qd.v[x];
             }
@@ -925,7 +937,7 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
             Q qd;
 #pragma unroll
             for (int x = 0; x < 4; ++x) qd.v[x] = v[q][x][y];
-            *reinterpret_cast<Q *>(&s[bq[q] ^ kb[y]]) = qd;
+            lds_st(s, bq[q] ^ kb[y], qd);
           }
         }
       }
@@ -935,13 +947,13 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
       using P2 = Pack<T, 2>;
       constexpr int UN = E / 32;
       for (int un = tid; un < UN; un += NT) {
-        const int b = lds_swz<SWZ>(((un & 1) << 1) | lane_deposit(un >> 1, op.qlanes, M - 3));
+        const uint32_t b = lds_swz<SWZ>(((un & 1) << 1) | lane_deposit(un >> 1, op.qlanes, M - 3)) * (uint32_t)sizeof(T);
         T v[2][4][4];
 #pragma unroll
         for (int x = 0; x < 4; ++x)
 #pragma unroll
           for (int y = 0; y < 4; ++y) {
-            const P2 p = *reinterpret_cast<const P2 *>(&s[b ^ ka[x] ^ kb[y]]);
+            const P2 p = lds_ld<P2>(s, b ^ ka[x] ^ kb[y]);
             v[0][x][y] = p.v[0];
             v[1][x][y] = p.v[1];
           }
@@ -954,23 +966,23 @@ __device__ __forceinline__ void tile_ops(T *s, const TileOp *__restrict__ ops, i
               P2 p;
               p.v[0] = v[0][x][y];
               p.v[1] = v[1][x][y];
-              *reinterpret_cast<P2 *>(&s[b ^ ka[x] ^ kb[y]]) = p;
+              lds_st(s, b ^ ka[x] ^ kb[y], p);
             }
       }
     } else {
       for (int g = tid; g < E / 16; g += NT) {
-        const int b = SWZ ? lds_swz<true>(lane_deposit(g, op.lanes, M - 2)) : ins_zero(ins_zero(g, ja), jb);
+        const uint32_t b = (SWZ ? lds_swz<true>(lane_deposit(g, op.lanes, M - 2)) : ins_zero(ins_zero(g, ja), jb)) * (uint32_t)sizeof(T);
         T v[1][4][4];
 #pragma unroll
         for (int x = 0; x < 4; ++x)
 #pragma unroll
-          for (int y = 0; y < 4; ++y) v[0][x][y] = s[b ^ ka[x] ^ kb[y]];
+          for (int y = 0; y < 4; ++y) v[0][x][y] = lds_ld<T>(s, b ^ ka[x] ^ kb[y]);
         stage_apply<T, GEN2Q, 1>(v, ja, mem, nm, prm);
 #pragma unroll
         for (int x = 0; x < 4; ++x)
 #pragma unroll
           for (int y = 0; y < 4; ++y)
-            if (x | y) s[b ^ ka[x] ^ kb[y]] = v[0][x][y];
+            if (x | y) lds_st(s, b ^ ka[x] ^ kb[y], v[0][x][y]);
       }
     }
     __syncthreads();
@@ -1088,12 +1100,12 @@ __device__ __forceinline__ void stage_io(T *__restrict__ a, uint64_t base, int w
   const TileOp *mem = ops + at + 1;
   const int nm = op.k;
   const int tid = threadIdx.x;
-  int ka[4], kb[4];
+  uint32_t ka[4], kb[4];  // LDS byte offsets of the components along ja, jb
   uint64_t ha[4], hb[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    ka[u] = lds_swz<SWZ>(u << (2 * ja));
-    kb[u] = lds_swz<SWZ>(u << (2 * jb));
+    ka[u] = lds_swz<SWZ>(u << (2 * ja)) * (uint32_t)sizeof(T);
+    kb[u] = lds_swz<SWZ>(u << (2 * jb)) * (uint32_t)sizeof(T);
     ha[u] = (uint64_t)u << (2 * geo.tpos[ja]);
     hb[u] = (uint64_t)u << (2 * geo.tpos[jb]);
   }
@@ -1104,7 +1116,7 @@ __device__ __forceinline__ void stage_io(T *__restrict__ a, uint64_t base, int w
     const int un = tid;
     if (un < E / 32) {
       const int bt = ((un & 1) << 1) | lane_deposit(un >> 1, op.qlanes, M - 3);
-      const int lb = lds_swz<SWZ>(bt);
+      const uint32_t lb = lds_swz<SWZ>(bt) * (uint32_t)sizeof(T);
       T *hp = a + base + tile_offset<T, M>(bt, w, lomask, geo);
       T v[2][4][4];
 #pragma unroll
@@ -1113,7 +1125,7 @@ __device__ __forceinline__ void stage_io(T *__restrict__ a, uint64_t base, int w
         for (int y = 0; y < 4; ++y) {
           P2 p;
           if (DIR == 1) {
-            p = *reinterpret_cast<const P2 *>(&s[lb ^ ka[x] ^ kb[y]]);
+            p = lds_ld<P2>(s, lb ^ ka[x] ^ kb[y]);
           } else {
             using V2 = T __attribute__((ext_vector_type(2)));
             const V2 *qp = reinterpret_cast<const V2 *>(hp + ha[x] + hb[y]);
@@ -1133,7 +1145,7 @@ __device__ __forceinline__ void stage_io(T *__restrict__ a, uint64_t base, int w
             P2 p;
             p.v[0] = v[0][x][y];
             p.v[1] = v[1][x][y];
-            *reinterpret_cast<P2 *>(&s[lb ^ ka[x] ^ kb[y]]) = p;
+            lds_st(s, lb ^ ka[x] ^ kb[y], p);
           } else {
             using V2 = T __attribute__((ext_vector_type(2)));
             V2 q;
@@ -1146,19 +1158,19 @@ __device__ __forceinline__ void stage_io(T *__restrict__ a, uint64_t base, int w
     }
     return;
   }
-  int lb[GPT];
+  uint32_t lb[GPT];
   T *hp[GPT];
   T v[GPT][4][4];
 #pragma unroll
   for (int q = 0; q < GPT; ++q) {
     const int g = tid + q * NT;
     const int bt = SWZ ? lane_deposit(g, op.lanes, M - 2) : ins_zero(ins_zero(g, ja), jb);
-    lb[q] = lds_swz<SWZ>(bt);
+    lb[q] = lds_swz<SWZ>(bt) * (uint32_t)sizeof(T);
     hp[q] = a + base + tile_offset<T, M>(bt, w, lomask, geo);
     if (ja == 0) {
 #pragma unroll
       for (int y = 0; y < 4; ++y) {
-        const Q qd = DIR == 1 ? lds_get_quad(s, lb[q] ^ kb[y]) : ldq<T, NTM>(hp[q] + hb[y]);
+        const Q qd = DIR == 1 ? lds_ld<Q>(s, lb[q] ^ kb[y]) : ldq<T, NTM>(hp[q] + hb[y]);
 #pragma unroll
         for (int x = 0; x < 4; ++x) v[q][x][y] = qd.v[x];
       }
@@ -1167,7 +1179,7 @@ __device__ __forceinline__ void stage_io(T *__restrict__ a, uint64_t base, int w
       for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int y = 0; y < 4; ++y)
-          v[q][x][y] = DIR == 1 ? s[lb[q] ^ ka[x] ^ kb[y]]
+          v[q][x][y] = DIR == 1 ? lds_ld<T>(s, lb[q] ^ ka[x] ^ kb[y])
                                 : NTM ? __builtin_nontemporal_load(hp[q] + ha[x] + hb[y]) : hp[q][ha[x] + hb[y]];
     }
   }
@@ -1180,7 +1192,7 @@ __device__ __forceinline__ void stage_io(T *__restrict__ a, uint64_t base, int w
         Q qd;
 #pragma unroll
         for (int x = 0; x < 4; ++x) qd.v[x] = v[q][x][y];
-        if (DIR == 0) lds_put_quad(s, lb[q] ^ kb[y], qd);
+        if (DIR == 0) lds_st(s, lb[q] ^ kb[y], qd);
         else stq<T, NTM>(hp[q] + hb[y], qd);
       }
     } else {
@@ -1188,7 +1200,7 @@ __device__ __forceinline__ void stage_io(T *__restrict__ a, uint64_t base, int w
       for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int y = 0; y < 4; ++y) {
-          if (DIR == 0) s[lb[q] ^ ka[x] ^ kb[y]] = v[q][x][y];
+          if (DIR == 0) lds_st(s, lb[q] ^ ka[x] ^ kb[y], v[q][x][y]);
           else if (NTM) __builtin_nontemporal_store(v[q][x][y], hp[q] + ha[x] + hb[y]);
           else hp[q][ha[x] + hb[y]] = v[q][x][y];
         }
