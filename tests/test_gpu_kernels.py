@@ -416,12 +416,12 @@ def test_fused_pass_two_qubit_ops_on_low_digits(prec, m, qs):
 @pytest.mark.parametrize('prec', PRECS)
 @pytest.mark.parametrize('m,seed', [(5, 1), (6, 2), (6, 3), (7, 4), (7, 5)])
 def test_fused_pass_stages_of_random_operations(prec, m, seed):
-    """A pass of 48 random operations on 4 qubits of the tile (single-qubit
-    PTMs, scaled and plain CX, dense two-qubit PTMs, repeats on one qubit
-    between operations on others): the library regroups them into
-    two-digit register stages, moving operations past commuting ones and
-    multiplying lone single-qubit runs; results must equal one kernel per
-    operation in the given order."""
+    """A pass of 48 random operations on 4 qubits of the tile (unital and
+    non-unital single-qubit PTMs, scaled and plain CX, dense two-qubit PTMs,
+    repeats on one qubit between operations on others): the library
+    regroups them into two-digit register stages, moving operations past
+    commuting ones and multiplying lone single-qubit runs; results must
+    equal one kernel per operation in the given order."""
     from dmsim.program import Program
     n = 9
     rng = np.random.default_rng(100 + seed)
@@ -434,6 +434,11 @@ def test_fused_pass_stages_of_random_operations(prec, m, seed):
         if r < 0.45:
             q = int(rng.choice(act))
             P = ptm.depolarizing1(1e-3) @ ptm.gate_ptm('u3', list(rng.uniform(-3, 3, 3)))
+            if r < 0.1:
+                # amplitude damping: the Z row has an identity component
+                gam = float(rng.uniform(0.01, 0.2))
+                P = np.diag([1.0, np.sqrt(1 - gam), np.sqrt(1 - gam), 1 - gam]) @ P
+                P[3, 0] = gam
             sub.append((native.SUB_1Q, [q], ptm.rows3(P)))
             plain.one_qubit(q, P)
         else:
