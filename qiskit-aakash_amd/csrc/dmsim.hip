@@ -1649,7 +1649,7 @@ __global__ __launch_bounds__(1024) void k_circuit_lds(const BatchCirc *__restric
 enum { BP_INIT = 0, BP_TILE = 1, BP_PROBS = 2 };
 
 struct BatchPass {
-  TileGeom geo;       // BP_TILE: the pass's 6-digit tile
+  TileGeom geo;       // BP_TILE: the pass's tile (6 digits, or 7 in FP32)
   int32_t kind;
   int32_t circ;       // HBM-tier circuit index (clock stamps)
   int32_t n;          // register size
@@ -1673,13 +1673,28 @@ constexpr int kQueueThreads = 256;
 constexpr int kQueueTile = 6;          // tile digits of HBM-tier passes
 constexpr int kQueueMaxQubits = 12;    // 2^12 FP64 probabilities fill the LDS buffer
 constexpr int kQueueLdsBytes = 32768;  // one FP64 6-digit tile, or 2^12 FP64 values
+// FP32 passes whose 6-digit tile would leave runs of 4^2 coefficients take
+// 7-digit tiles (64 KB of LDS, 512 threads, as the one-tile kernel), run by
+// a second launch per round over those items only.
+constexpr int kQueueTile7 = 7;
+constexpr int kQueueThreads7 = 512;
+template <int M>
+struct QueueShape {
+  static constexpr int threads = M == kQueueTile7 ? kQueueThreads7 : kQueueThreads;
+  static constexpr int lds = M == kQueueTile7 ? 65536 : kQueueLdsBytes;
+};
 
-template <typename T, bool SWZ>
+template <typename T, bool SWZ, int M>
 __device__ __forceinline__ void queue_tiles(T *s, T *a, const BatchPass &bp, uint32_t t0, uint32_t nt,
                                             const TileOp *__restrict__ tops, const T *__restrict__ prm) {
-  constexpr int M = kQueueTile;
+  constexpr int NT = QueueShape<M>::threads;
   constexpr int E = 1 << (2 * M);
-  constexpr int QPT = E / 4 / kQueueThreads;
+  constexpr int QPT = E / 4 / NT;
+  constexpr bool VEC = E / NT >= 32;
+  // 7-digit passes hold single-qubit and CX members only (the host refuses
+  // others): the 16x16 member's registers would spill at 128 per lane
+  constexpr bool GEN2Q = M != kQueueTile7;
+  static_assert(E * sizeof(T) <= QueueShape<M>::lds, "the tile fits the round kernel's LDS");
   using Q = Pack<T, 4>;
   const int tid = threadIdx.x;
   const int w = bp.geo.w;
@@ -1694,30 +1709,30 @@ __device__ __forceinline__ void queue_tiles(T *s, T *a, const BatchPass &bp, uin
     // the first and last stage read and write HBM themselves where
     // stage_io allows (bp.io), as in the one-tile kernel
     if ((bp.io & 3) == 3 && bp.last_at == 0) {
-      stage_io<T, M, true, kQueueThreads, SWZ, false, 2, false>(a, base, w, lomask, bp.geo, s, ops, 0, prm);
+      stage_io<T, M, GEN2Q, NT, SWZ, VEC, 2, false>(a, base, w, lomask, bp.geo, s, ops, 0, prm);
       continue;
     }
     int i0 = 0;
     if (bp.io & 1) {
-      stage_io<T, M, true, kQueueThreads, SWZ, false, 0, false>(a, base, w, lomask, bp.geo, s, ops, 0, prm);
+      stage_io<T, M, GEN2Q, NT, SWZ, VEC, 0, false>(a, base, w, lomask, bp.geo, s, ops, 0, prm);
       i0 = 1 + ops[0].k;
     } else {
       Q buf[QPT];
 #pragma unroll
       for (int i = 0; i < QPT; ++i)
-        buf[i] = ldp<Q>(a + base + tile_offset<T, M>(4 * (tid + i * kQueueThreads), w, lomask, bp.geo));
+        buf[i] = ldp<Q>(a + base + tile_offset<T, M>(4 * (tid + i * NT), w, lomask, bp.geo));
 #pragma unroll
-      for (int i = 0; i < QPT; ++i) lds_put_quad(s, lds_swz<SWZ>(4 * (tid + i * kQueueThreads)), buf[i]);
+      for (int i = 0; i < QPT; ++i) lds_put_quad(s, lds_swz<SWZ>(4 * (tid + i * NT)), buf[i]);
     }
     __syncthreads();
     const int i1 = (bp.io & 2) ? bp.last_at : bp.nops;
-    tile_ops<T, M, false, true, kQueueThreads, SWZ>(s, ops + i0, i1 - i0, prm);
+    tile_ops<T, M, false, GEN2Q, NT, SWZ>(s, ops + i0, i1 - i0, prm);
     if (bp.io & 2) {
-      stage_io<T, M, true, kQueueThreads, SWZ, false, 1, false>(a, base, w, lomask, bp.geo, s, ops, bp.last_at, prm);
+      stage_io<T, M, GEN2Q, NT, SWZ, VEC, 1, false>(a, base, w, lomask, bp.geo, s, ops, bp.last_at, prm);
     } else {
 #pragma unroll
       for (int i = 0; i < QPT; ++i) {
-        const int c = tid + i * kQueueThreads;
+        const int c = tid + i * NT;
         stp(a + base + tile_offset<T, M>(4 * c, w, lomask, bp.geo), lds_get_quad(s, lds_swz<SWZ>(4 * c)));
       }
     }
@@ -1725,24 +1740,24 @@ __device__ __forceinline__ void queue_tiles(T *s, T *a, const BatchPass &bp, uin
   }
 }
 
-// The items [i0, i1) of one round, grid-stride over workgroups.
-template <typename T>
-__global__ __launch_bounds__(kQueueThreads) void k_batch_round(const BatchPass *__restrict__ passes,
-                                                               const BatchItem *__restrict__ items, uint32_t i0,
-                                                               uint32_t i1, const TileOp *__restrict__ tops,
-                                                               const T *__restrict__ prm, T *__restrict__ arena,
-                                                               double *__restrict__ probs,
-                                                               uint64_t *__restrict__ stamps) {
-  __shared__ __align__(32) unsigned char raw[kQueueLdsBytes];
+// The items [i0, i1) of one round, grid-stride over workgroups.  M = 6:
+// the round's 6-digit tile items and its INIT and PROBS items; M = 7: its
+// 7-digit tile items (FP32).
+template <typename T, int M>
+__global__ __launch_bounds__(QueueShape<M>::threads) __attribute__((amdgpu_waves_per_eu(M == kQueueTile7 ? 4 : 1))) void k_batch_round(
+    const BatchPass *__restrict__ passes, const BatchItem *__restrict__ items, uint32_t i0, uint32_t i1,
+    const TileOp *__restrict__ tops, const T *__restrict__ prm, T *__restrict__ arena, double *__restrict__ probs,
+    uint64_t *__restrict__ stamps) {
+  __shared__ __align__(32) unsigned char raw[QueueShape<M>::lds];
   const int tid = threadIdx.x;
   for (uint32_t t = i0 + blockIdx.x; t < i1; t += gridDim.x) {
     const BatchItem it = items[t];
     const BatchPass &bp = passes[it.pass];
     T *a = arena + bp.state_off;
-    if (bp.kind == BP_TILE) {
+    if (M == kQueueTile7 || bp.kind == BP_TILE) {
       T *s = reinterpret_cast<T *>(raw);
-      if (bp.geo.ident) queue_tiles<T, false>(s, a, bp, it.tile0, it.ntiles, tops, prm);
-      else queue_tiles<T, true>(s, a, bp, it.tile0, it.ntiles, tops, prm);
+      if (bp.geo.ident) queue_tiles<T, false, M>(s, a, bp, it.tile0, it.ntiles, tops, prm);
+      else queue_tiles<T, true, M>(s, a, bp, it.tile0, it.ntiles, tops, prm);
     } else if (bp.kind == BP_INIT) {
       if (tid == 0 && it.tile0 == 0) stamp_now(&stamps[2 * bp.circ]);
       const T amp = (T)ldexp(1.0, -bp.n);
@@ -1763,8 +1778,8 @@ __global__ __launch_bounds__(kQueueThreads) void k_batch_round(const BatchPass *
       // the butterfly on bit q applies (u, w) -> (u + e w, u - e w), and the
       // outcome index has qubit 0 as its most significant bit, as v does
       double *v = reinterpret_cast<double *>(raw);
-      const int n = bp.n, M = 1 << n;
-      for (int x = tid; x < M; x += kQueueThreads) {
+      const int n = bp.n, nv = 1 << n;
+      for (int x = tid; x < nv; x += kQueueThreads) {
         uint64_t idx = 0;
         for (int q = 0; q < n; ++q)
           if ((x >> q) & 1) idx |= 3ull << (2 * q);
@@ -1772,7 +1787,7 @@ __global__ __launch_bounds__(kQueueThreads) void k_batch_round(const BatchPass *
       }
       __syncthreads();
       for (int q = 0; q < n; ++q) {
-        for (int x = tid; x < M / 2; x += kQueueThreads) {
+        for (int x = tid; x < nv / 2; x += kQueueThreads) {
           const int lo = x & ((1 << q) - 1);
           const int b = ((x >> q) << (q + 1)) | lo;
           const double u = v[b], wv = v[b | (1 << q)];
@@ -1781,7 +1796,7 @@ __global__ __launch_bounds__(kQueueThreads) void k_batch_round(const BatchPass *
         }
         __syncthreads();
       }
-      for (int x = tid; x < M; x += kQueueThreads) probs[bp.out_off + x] = v[x];
+      for (int x = tid; x < nv; x += kQueueThreads) probs[bp.out_off + x] = v[x];
       if (tid == 0) stamp_now(&stamps[2 * bp.circ + 1]);
     }
     __syncthreads();  // the LDS buffer is free before the next item
@@ -3114,9 +3129,12 @@ struct dm_batch {
   std::vector<int> large;
   void *d_arena = nullptr, *d_passes = nullptr, *d_items = nullptr, *d_qtops = nullptr, *d_qvals = nullptr;
   std::vector<uint32_t> round_start;  // items of round r: [round_start[r], round_start[r + 1])
+  // the 7-digit tile items of round r: [round_split[r], round_start[r + 1])
+  std::vector<uint32_t> round_split;
   uint32_t nitems = 0;
   int64_t npasses = 0;
-  int queue_grid = 0;  // resident workgroups of k_batch_round on the device
+  int queue_grid = 0;   // resident workgroups of k_batch_round<T, 6> on the device
+  int queue_grid7 = 0;  // and of k_batch_round<float, 7>
   hipStream_t master = nullptr, side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr, ev0 = nullptr, ev1 = nullptr;
   double *d_probs = nullptr;
@@ -3241,11 +3259,15 @@ int queue_op_passes(const dm_state *h, const dm_op &op, const double *params, in
       if (op.param < 0 || op.param >= nparams) return fail(DM_ERR_ARG, "circuit %d op %d: bad parameter offset", ci, i);
       rc = parse_fused(h, params + op.param, nparams - op.param, i, &fp);
       if (rc) return rc;
-      if (fp.m != kQueueTile)
-        return fail(DM_ERR_ARG, "circuit %d op %d: batched circuits take %d-digit fused tiles, not %d", ci, i,
-                    kQueueTile, fp.m);
-      for (const TileOp &t : fp.ops)
+      if (fp.m != kQueueTile && !(fp.m == kQueueTile7 && h->dtype == DM_FP32))
+        return fail(DM_ERR_ARG, "circuit %d op %d: batched circuits take %d-digit fused tiles (or %d in FP32), not %d",
+                    ci, i, kQueueTile, kQueueTile7, fp.m);
+      for (const TileOp &t : fp.ops) {
         if (t.kind == TOP_DENSE) return fail(DM_ERR_ARG, "circuit %d op %d: dense tile operations are not batched", ci, i);
+        if (t.kind == TOP_2Q && fp.m == kQueueTile7)
+          return fail(DM_ERR_ARG, "circuit %d op %d: batched %d-digit passes take single-qubit and CX operations only",
+                      ci, i, kQueueTile7);
+      }
       break;
     }
     case DM_OP_MEMORY_NOISE: {
@@ -3290,15 +3312,28 @@ int enqueue_batch(dm_batch *b) {
                        static_cast<const T *>(b->d_vals), b->d_probs, b->d_stamps + 2 * (size_t)lo);
     if (int rc = last_launch("k_circuit_lds")) return rc;
   }
-  // HBM tier on the master stream: one launch per round, in order
+  // HBM tier on the master stream: one launch per round, in order (two
+  // when the round has 7-digit tile items)
   for (size_t r = 0; r + 1 < b->round_start.size(); ++r) {
-    const uint32_t i0 = b->round_start[r], i1 = b->round_start[r + 1];
-    const int grid = (int)std::min<uint32_t>(i1 - i0, (uint32_t)b->queue_grid);
-    hipLaunchKernelGGL((k_batch_round<T>), dim3(grid), dim3(kQueueThreads), 0, b->master,
-                       static_cast<const BatchPass *>(b->d_passes), static_cast<const BatchItem *>(b->d_items), i0,
-                       i1, static_cast<const TileOp *>(b->d_qtops), static_cast<const T *>(b->d_qvals),
-                       static_cast<T *>(b->d_arena), b->d_probs, b->d_stamps + 2 * b->small.size());
-    if (int rc = last_launch("k_batch_round")) return rc;
+    const uint32_t i0 = b->round_start[r], i1 = b->round_split[r], i2 = b->round_start[r + 1];
+    if (i1 > i0) {
+      const int grid = (int)std::min<uint32_t>(i1 - i0, (uint32_t)b->queue_grid);
+      hipLaunchKernelGGL((k_batch_round<T, kQueueTile>), dim3(grid), dim3(kQueueThreads), 0, b->master,
+                         static_cast<const BatchPass *>(b->d_passes), static_cast<const BatchItem *>(b->d_items), i0,
+                         i1, static_cast<const TileOp *>(b->d_qtops), static_cast<const T *>(b->d_qvals),
+                         static_cast<T *>(b->d_arena), b->d_probs, b->d_stamps + 2 * b->small.size());
+      if (int rc = last_launch("k_batch_round")) return rc;
+    }
+    if constexpr (sizeof(T) == 4) {
+      if (i2 > i1) {
+        const int grid = (int)std::min<uint32_t>(i2 - i1, (uint32_t)b->queue_grid7);
+        hipLaunchKernelGGL((k_batch_round<float, kQueueTile7>), dim3(grid), dim3(kQueueThreads7), 0, b->master,
+                           static_cast<const BatchPass *>(b->d_passes), static_cast<const BatchItem *>(b->d_items), i1,
+                           i2, static_cast<const TileOp *>(b->d_qtops), static_cast<const T *>(b->d_qvals),
+                           static_cast<T *>(b->d_arena), b->d_probs, b->d_stamps + 2 * b->small.size());
+        if (int rc = last_launch("k_batch_round")) return rc;
+      }
+    }
   }
   HIP_TRY(hipEventRecord(b->join, b->side));
   HIP_TRY(hipStreamWaitEvent(b->master, b->join, 0));
@@ -3428,21 +3463,28 @@ static int batch_create_t(dm_batch *b, const int64_t *op_start, const dm_op *ops
     for (auto &ids : chain) rounds = std::max(rounds, ids.size());
     for (size_t r = 0; r < rounds; ++r) {
       b->round_start.push_back((uint32_t)items.size());
-      for (int j : order) {
-        if (r >= chain[j].size()) continue;
-        const int pi = chain[j][r];
-        const BatchPass &bp = passes[pi];
-        const uint64_t units = bp.kind == BP_PROBS ? 1 : pow4(bp.n - kQueueTile);
-        const uint64_t per = std::max<uint64_t>(1, (units + items_per_pass - 1) / items_per_pass);
-        for (uint64_t t0 = 0; t0 < units; t0 += per) {
-          BatchItem it;
-          it.pass = (uint32_t)pi;
-          it.tile0 = (uint32_t)t0;
-          it.ntiles = (uint32_t)std::min<uint64_t>(per, units - t0);
-          it.pad = 0;
-          items.push_back(it);
+      // the round's 6-digit tile, INIT and PROBS items, then its 7-digit ones
+      for (int seven = 0; seven < 2; ++seven) {
+        if (seven) b->round_split.push_back((uint32_t)items.size());
+        for (int j : order) {
+          if (r >= chain[j].size()) continue;
+          const int pi = chain[j][r];
+          const BatchPass &bp = passes[pi];
+          const bool is7 = bp.kind == BP_TILE && bp.geo.m == kQueueTile7;
+          if (is7 != (seven == 1)) continue;
+          const uint64_t units =
+              bp.kind == BP_PROBS ? 1 : pow4(bp.n - (bp.kind == BP_TILE ? bp.geo.m : kQueueTile));
+          const uint64_t per = std::max<uint64_t>(1, (units + items_per_pass - 1) / items_per_pass);
+          for (uint64_t t0 = 0; t0 < units; t0 += per) {
+            BatchItem it;
+            it.pass = (uint32_t)pi;
+            it.tile0 = (uint32_t)t0;
+            it.ntiles = (uint32_t)std::min<uint64_t>(per, units - t0);
+            it.pad = 0;
+            items.push_back(it);
+          }
+          if (items.size() > (size_t)UINT32_MAX / 2) return fail(DM_ERR_ARG, "batch: too many work items");
         }
-        if (items.size() > (size_t)UINT32_MAX / 2) return fail(DM_ERR_ARG, "batch: too many work items");
       }
     }
     if (rounds) b->round_start.push_back((uint32_t)items.size());
@@ -3478,11 +3520,21 @@ static int batch_create_t(dm_batch *b, const int64_t *op_start, const dm_op *ops
     if (int rc = upload(&b->d_qtops, qtops.data(), qtops.size() * sizeof(TileOp))) return rc;
     if (int rc = upload(&b->d_qvals, tv.data(), tv.size() * sizeof(T))) return rc;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_round<T>, kQueueThreads, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_round<T, kQueueTile>, kQueueThreads, 0) !=
+            hipSuccess ||
         per_cu < 1)
       per_cu = 1;
     b->queue_grid = cu * per_cu;
-    b->launches += (int64_t)b->round_start.size() - 1;
+    if constexpr (sizeof(T) == 4) {
+      int per7 = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per7, k_batch_round<float, kQueueTile7>, kQueueThreads7, 0) !=
+              hipSuccess ||
+          per7 < 1)
+        per7 = 1;
+      b->queue_grid7 = cu * per7;
+    }
+    for (size_t r = 0; r + 1 < b->round_start.size(); ++r)
+      b->launches += (b->round_split[r] > b->round_start[r] ? 1 : 0) + (b->round_start[r + 1] > b->round_split[r] ? 1 : 0);
   }
   HIP_TRY(hipMalloc(&b->d_probs, std::max<int64_t>(1, b->nprobs) * sizeof(double)));
   HIP_TRY(hipMalloc(&b->d_stamps, 2 * (size_t)b->ncirc * sizeof(uint64_t)));

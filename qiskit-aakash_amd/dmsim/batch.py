@@ -6,9 +6,11 @@ gate, dm_simulator.py:1172-1177), and finally an ensemble Z measurement of
 all qubits with polarisation retention ``depolarization_factor``.  Small
 registers (<= ``lds_max_qubits``) run whole inside one workgroup's LDS; the
 others keep their state in one HBM arena and run as fused 6-digit tile
-passes, one launch per round (pass r of every circuit), each pass split into
-at most ``items_per_pass`` work items; the batch is captured once into a
-hipGraph and replayed (``dm_batch_*`` in dmsim.h).
+passes (7-digit in FP32 where 6 digits would leave contiguous runs of only
+4^2 coefficients), one launch per round (pass r of every circuit; two when
+the round has 7-digit passes), each pass split into at most
+``items_per_pass`` work items; the batch is captured once into a hipGraph
+and replayed (``dm_batch_*`` in dmsim.h).
 
 ``autotune`` picks the LDS cutoff width, the LDS workgroup size and the
 HBM tier's items per pass by timing the candidates on the batch itself; the
@@ -27,8 +29,10 @@ from .program import Program
 
 _TUNED = {}
 
-# tile digits of the HBM tier's passes (dm_batch_create takes 6-digit fused tiles)
+# tile digits of the HBM tier's passes (dm_batch_create takes 6-digit fused
+# tiles, and 7-digit ones in FP32)
 QUEUE_TILE = 6
+QUEUE_TILE_FP32_LONG = 7
 # autotuner candidates
 LDS_MAX_CHOICES = (5, 6, 7)
 LDS_THREAD_CHOICES = (256, 512, 1024)
@@ -53,13 +57,26 @@ def circuit_program(n, gates, decay_factor=1.0, decoherence_factor=1.0, thermal_
     return prog
 
 
+def batch_tile(n, qs, precision):
+    """Tile digits of a batched pass over block ``qs``: 6, or in FP32 7 when
+    the 6-digit tile would leave contiguous runs of only 4^2 coefficients
+    and 7 digits give longer ones (registers of at least 7 qubits)."""
+    from . import fusion
+    if precision == 'fp32' and n >= QUEUE_TILE_FP32_LONG and len(qs) > 1:
+        pos = [n - 1 - q for q in qs]
+        w6 = fusion.run_digits(n, pos, QUEUE_TILE)
+        if w6 < 3 and fusion.run_digits(n, pos, QUEUE_TILE_FP32_LONG) > w6:
+            return QUEUE_TILE_FP32_LONG
+    return QUEUE_TILE
+
+
 def fused_circuit_program(n, gates, decay_factor=1.0, decoherence_factor=1.0, thermal_factor=1.0,
-                          gate_depolarization=0.0):
+                          gate_depolarization=0.0, precision='fp64'):
     """The same circuit as ``circuit_program`` as fused tile passes: the
     memory noise of every level enters the gate stream as one 1-qubit
     channel per qubit, and the greedy planner groups everything into blocks
-    of at most 4 qubits, each pass a 6-digit tile (registers of at least 6
-    qubits)."""
+    of at most 4 qubits, each pass a tile of ``batch_tile`` digits
+    (registers of at least 6 qubits)."""
     from . import fusion
     noisy = []
     mem = ptm.memory_noise(decoherence_factor, thermal_factor, decay_factor)
@@ -68,7 +85,10 @@ def fused_circuit_program(n, gates, decay_factor=1.0, decoherence_factor=1.0, th
         noisy += fusion.noisy_gates([gate], gate_depolarization)
         if not identity:
             noisy += [fusion.NoisyGate((q,), mem, 'u3') for q in range(n)]
-    prog, _ = fusion.fused_program(noisy, n, fusion.KMAX['greedy'], 'greedy', None, 'seq', tile=QUEUE_TILE)
+    blocks = fusion.plan(noisy, n, fusion.KMAX['greedy'], 'greedy', None, tiles=False, precision=precision)
+    prog = Program(n)
+    for qs, mem_ids in blocks:
+        fusion.add_block(prog, noisy, qs, mem_ids, 'seq', batch_tile(n, qs, precision), precision)
     return prog
 
 
@@ -147,10 +167,13 @@ def from_workload(specs, precision='fp64', lds_max_qubits=7, programs=None, **kw
     circuits = []
     for i, (n, gates, dr, pr) in enumerate(specs):
         fused = n > lds_max_qubits
-        key = (i, fused)
+        key = (i, fused, precision if fused else None)
         prog = programs.get(key) if programs is not None else None
         if prog is None:
-            prog = (fused_circuit_program if fused else circuit_program)(n, gates, decay_factor=1.0 - dr)
+            if fused:
+                prog = fused_circuit_program(n, gates, decay_factor=1.0 - dr, precision=precision)
+            else:
+                prog = circuit_program(n, gates, decay_factor=1.0 - dr)
             if programs is not None:
                 programs[key] = prog
         circuits.append((n, prog, 1.0 - pr))

@@ -47,21 +47,48 @@ def test_batch_split_choices_agree(lmax, threads, ipp):
         assert np.max(np.abs(g - w)) <= 1e-13
 
 
-def b_circuits(specs):
-    return [(n, batch.fused_circuit_program(n, gates, decay_factor=1.0 - dr)) for n, gates, dr, _ in specs]
+def pass_tiles(prog):
+    """Tile digits of each pass of a fused circuit program (single-gate
+    operations run as 6-digit passes)."""
+    from dmsim import native
+    arr, params = prog.compile()
+    return [int(params[arr[i].param]) if arr[i].kind == native.OP_FUSED else 6 for i in range(len(prog))]
+
+
+def round_launches(progs):
+    """Launches of one batch run: round r holds INIT (r = 0), pass r - 1 or
+    PROBS of every circuit; one launch for its 6-digit, INIT and PROBS items
+    and one for its 7-digit items."""
+    tiles = [pass_tiles(p) for p in progs]
+    rounds = max(len(t) for t in tiles) + 2
+    n = 0
+    for r in range(rounds):
+        six = seven = False
+        for t in tiles:
+            if r == 0 or r == len(t) + 1:
+                six = True
+            elif r <= len(t):
+                six, seven = six or t[r - 1] == 6, seven or t[r - 1] == 7
+        n += six + seven
+    return n
 
 
 def test_batching_workload_reduced_against_oracle():
     """The batching workload's generator (widths 6-10 here, depths 20-60,
     per-circuit decay and measurement rates) with every HBM-tier circuit in
-    the round kernels, graph replayed three times, FP64 and FP32."""
+    the round kernels, graph replayed three times, FP64 and FP32 (whose
+    passes over high qubits take 7-digit tiles)."""
     specs = workloads.batch(24, 3000, n_range=(6, 10), depth_range=(20, 60))
     wants = [oracle_probs(n, gates, dr, pr) for n, gates, dr, pr in specs]
     for precision, tol in (('fp64', 1e-12), ('fp32', 2e-5)):
         b = batch.from_workload(specs, precision, lds_max_qubits=5, lds_threads=256, items_per_pass=64)
         info = b.info()
         assert info['hbm_circuits'] == 24
-        assert info['launches'] == max(len(c[1]) for c in b_circuits(specs)) + 2
+        progs = [batch.fused_circuit_program(n, gates, decay_factor=1.0 - dr, precision=precision)
+                 for n, gates, dr, _ in specs]
+        sevens = sum(t == 7 for p in progs for t in pass_tiles(p))
+        assert (sevens > 0) == (precision == 'fp32')
+        assert info['launches'] == round_launches(progs)
         for rep in range(3):
             probs, us, ms = b.run(True)
             for (n, _, _, _), got, want in zip(specs, probs, wants):
