@@ -60,12 +60,15 @@ def circuit_program(n, gates, decay_factor=1.0, decoherence_factor=1.0, thermal_
 def batch_tile(n, qs, precision):
     """Tile digits of a batched pass over block ``qs``: 6, or in FP32 7 when
     the 6-digit tile would leave contiguous runs of only 4^2 coefficients
-    and 7 digits give longer ones (registers of at least 7 qubits)."""
+    and 7 digits give longer ones, or runs of 4^3 that 7 digits make 4^4
+    (registers of at least 7 qubits; the single-state rule, measured faster
+    on the batch too)."""
     from . import fusion
     if precision == 'fp32' and n >= QUEUE_TILE_FP32_LONG and len(qs) > 1:
         pos = [n - 1 - q for q in qs]
         w6 = fusion.run_digits(n, pos, QUEUE_TILE)
-        if w6 < 3 and fusion.run_digits(n, pos, QUEUE_TILE_FP32_LONG) > w6:
+        w7 = fusion.run_digits(n, pos, QUEUE_TILE_FP32_LONG)
+        if (w6 < 3 and w7 > w6) or (w6 == 3 and w7 == 4):
             return QUEUE_TILE_FP32_LONG
     return QUEUE_TILE
 
