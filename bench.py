@@ -332,21 +332,42 @@ def tile_ops_mode(args):
     return out
 
 
+def parse_batch_choice(text):
+    """'fp64=6/256/256,fp32=5/256/64' -> {precision: (lds_max_qubits,
+    lds_threads, items_per_pass)}; '' -> {}."""
+    out = {}
+    for part in filter(None, (text or '').split(',')):
+        prec, _, val = part.partition('=')
+        if prec not in ('fp64', 'fp32'):
+            raise ValueError('--batch-choice: unknown precision %r' % prec)
+        vals = tuple(int(x) for x in val.split('/'))
+        if len(vals) != 3:
+            raise ValueError('--batch-choice: want lds_max_qubits/lds_threads/items_per_pass, got %r' % val)
+        out[prec] = vals
+    return out
+
+
 def batch_mode(args):
     """Batched circuits (U3/CX 70/30, memory noise after every gate, Z
-    ensemble at the end), FP64 and FP32, autotuned, hipGraph replay."""
+    ensemble at the end), FP64 and FP32, autotuned (or the --batch-choice
+    given for a precision), hipGraph replay."""
     from dmsim import batch
     lo, hi = (int(x) for x in args.batch_widths.split('-'))
     count = args.batch_circuits
     specs = workloads.batch(count, args.seed, n_range=(lo, hi))
+    fixed = parse_batch_choice(args.batch_choice)
     lines = {}
     programs = {}
     for prec in ('fp64', 'fp32'):
-        log('%s: autotuning ...' % prec)
-        t0 = time.perf_counter()
-        (lmax, thr, ipp), tune = batch.autotune(specs, prec, programs=programs)
-        log('  chose lds_max_qubits %d, %d threads, %d items per pass (%.1f s)' % (lmax, thr, ipp,
-                                                                                 time.perf_counter() - t0))
+        if prec in fixed:
+            (lmax, thr, ipp), tune = fixed[prec], {}
+            log('%s: lds_max_qubits %d, %d threads, %d items per pass (given)' % (prec, lmax, thr, ipp))
+        else:
+            log('%s: autotuning ...' % prec)
+            t0 = time.perf_counter()
+            (lmax, thr, ipp), tune = batch.autotune(specs, prec, programs=programs)
+            log('  chose lds_max_qubits %d, %d threads, %d items per pass (%.1f s)' % (lmax, thr, ipp,
+                                                                                     time.perf_counter() - t0))
         b = batch.from_workload(specs, prec, lds_max_qubits=lmax, lds_threads=thr, items_per_pass=ipp,
                                 programs=programs)
         info = b.info()
@@ -471,6 +492,9 @@ def parse_args(argv=None):
     ap.add_argument('--tile-ops', action='store_true', help='per-operation cost of tile passes (LDS counters)')
     ap.add_argument('--batch-circuits', type=int, default=512, help='batch config: circuits')
     ap.add_argument('--batch-widths', default='6-12', help='batch config: qubit range lo-hi')
+    ap.add_argument('--batch-choice', default='',
+                    help='batch config: fixed lds_max_qubits/lds_threads/items_per_pass per precision instead of '
+                         'autotuning, e.g. fp64=6/256/256,fp32=5/256/64')
     return ap.parse_args(argv)
 
 

@@ -31,3 +31,17 @@ def test_driver_command_plans_bounded_work():
     est_s = 11.0 * plan['plain_fp64'] + 0.9 * plan['fused_fp32'] + 20.0
     assert est_s < 120.0
     assert bench.planned_applications(bench.parse_args(['--all-paths']))['other_paths'] > 0
+
+
+def test_batch_choice_parsing():
+    """--batch-choice fixes the batch configuration per precision (no
+    autotuning for the precisions it names) and refuses malformed values."""
+    import pytest
+    sys.path.insert(0, ROOT)
+    import bench
+    assert bench.parse_batch_choice('') == {}
+    assert bench.parse_batch_choice('fp64=6/256/256,fp32=5/256/64') == {'fp64': (6, 256, 256), 'fp32': (5, 256, 64)}
+    with pytest.raises(ValueError):
+        bench.parse_batch_choice('fp16=6/256/256')
+    with pytest.raises(ValueError):
+        bench.parse_batch_choice('fp32=6/256')
