@@ -530,7 +530,7 @@ def planned_applications(args):
     accuracy comparison, then ``warmup`` + ``steps`` times."""
     return {'plain_fp64': args.plain_steps + PLAIN_WARMUP_GATES / max(1, args.gates),
             'fused_fp32': 1 + args.warmup + args.steps,
-            'other_paths': 5 * (1 + 2) if args.all_paths else 0}
+            'other_paths': 6 * (1 + 2) if args.all_paths else 0}
 
 
 # gates of the plain FP64 program run once before its timed applications
@@ -628,9 +628,12 @@ def headline(args):
         # the other forms of the same workload, 1 warm-up + 2 timed steps each,
         # reported on stderr after the headline line
         extra = {}
-        log('timing plain fp32, fused fp64, greedy dense-MFMA fp32 ...')
+        log('timing plain fp32, fused fp64 (FP32 plan and its own), greedy dense-MFMA fp32 ...')
         extra['plain_fp32'] = time_program(plain, n, 'fp32', 2, 1)
         extra['fused_fp64'] = time_program(fused, n, 'fp64', 2, 1)
+        # the program an FP64 state gets from the planner (FP64 costs and tiles)
+        fused64, _, _, _ = headline_plan(noisy, n, args.fusion, 'fp64')
+        extra['fused_fp64_own_plan'] = time_program(fused64, n, 'fp64', 2, 1)
         dense, _ = fusion.fused_program(noisy, n, 4, 'greedy', None, 'dense')
         extra['greedy_dense_fp32'] = time_program(dense, n, 'fp32', 2, 1)
         dense3, _ = fusion.fused_program(noisy, n, 3, 'greedy', None, 'dense')
