@@ -3137,6 +3137,9 @@ struct dm_batch {
   int queue_grid7 = 0;  // and of k_batch_round<float, 7>
   hipStream_t master = nullptr, side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr, ev0 = nullptr, ev1 = nullptr;
+  // a round's 7-digit launch runs beside its 6-digit launch (FP32)
+  hipStream_t side7 = nullptr;
+  hipEvent_t fork7 = nullptr, join7 = nullptr;
   double *d_probs = nullptr;
   uint64_t *d_stamps = nullptr;  // start / end clock per circuit: LDS tier first, then the HBM tier
   hipGraph_t graph = nullptr;
@@ -3312,10 +3315,17 @@ int enqueue_batch(dm_batch *b) {
                        static_cast<const T *>(b->d_vals), b->d_probs, b->d_stamps + 2 * (size_t)lo);
     if (int rc = last_launch("k_circuit_lds")) return rc;
   }
-  // HBM tier on the master stream: one launch per round, in order (two
-  // when the round has 7-digit tile items)
+  // HBM tier on the master stream: one launch per round, in order; a
+  // round with 7-digit tile items launches them on side7 beside its
+  // 6-digit launch (distinct circuits: independent items), joined before
+  // the next round
   for (size_t r = 0; r + 1 < b->round_start.size(); ++r) {
     const uint32_t i0 = b->round_start[r], i1 = b->round_split[r], i2 = b->round_start[r + 1];
+    const bool both = sizeof(T) == 4 && i1 > i0 && i2 > i1;
+    if (both) {
+      HIP_TRY(hipEventRecord(b->fork7, b->master));
+      HIP_TRY(hipStreamWaitEvent(b->side7, b->fork7, 0));
+    }
     if (i1 > i0) {
       const int grid = (int)std::min<uint32_t>(i1 - i0, (uint32_t)b->queue_grid);
       hipLaunchKernelGGL((k_batch_round<T, kQueueTile>), dim3(grid), dim3(kQueueThreads), 0, b->master,
@@ -3327,12 +3337,17 @@ int enqueue_batch(dm_batch *b) {
     if constexpr (sizeof(T) == 4) {
       if (i2 > i1) {
         const int grid = (int)std::min<uint32_t>(i2 - i1, (uint32_t)b->queue_grid7);
-        hipLaunchKernelGGL((k_batch_round<float, kQueueTile7>), dim3(grid), dim3(kQueueThreads7), 0, b->master,
-                           static_cast<const BatchPass *>(b->d_passes), static_cast<const BatchItem *>(b->d_items), i1,
-                           i2, static_cast<const TileOp *>(b->d_qtops), static_cast<const T *>(b->d_qvals),
+        hipLaunchKernelGGL((k_batch_round<float, kQueueTile7>), dim3(grid), dim3(kQueueThreads7), 0,
+                           both ? b->side7 : b->master, static_cast<const BatchPass *>(b->d_passes),
+                           static_cast<const BatchItem *>(b->d_items), i1, i2,
+                           static_cast<const TileOp *>(b->d_qtops), static_cast<const T *>(b->d_qvals),
                            static_cast<T *>(b->d_arena), b->d_probs, b->d_stamps + 2 * b->small.size());
         if (int rc = last_launch("k_batch_round")) return rc;
       }
+    }
+    if (both) {
+      HIP_TRY(hipEventRecord(b->join7, b->side7));
+      HIP_TRY(hipStreamWaitEvent(b->master, b->join7, 0));
     }
   }
   HIP_TRY(hipEventRecord(b->join, b->side));
@@ -3495,6 +3510,11 @@ static int batch_create_t(dm_batch *b, const int64_t *op_start, const dm_op *ops
   HIP_TRY(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
   HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
+  if (b->dtype == DM_FP32) {
+    HIP_TRY(hipStreamCreateWithFlags(&b->side7, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&b->fork7, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&b->join7, hipEventDisableTiming));
+  }
   HIP_TRY(hipEventCreate(&b->ev0));
   HIP_TRY(hipEventCreate(&b->ev1));
   auto upload = [](void **dst, const void *src, size_t bytes) -> int {
@@ -3906,14 +3926,15 @@ int dm_batch_destroy(dm_batch *b) {
   if (!b) return DM_OK;
   if (b->master) hipStreamSynchronize(b->master);
   if (b->side) hipStreamSynchronize(b->side);
+  if (b->side7) hipStreamSynchronize(b->side7);
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
   for (void *p : {b->d_circs, b->d_tops, b->d_vals, b->d_arena, b->d_passes, b->d_items, b->d_qtops, b->d_qvals,
                   (void *)b->d_probs, (void *)b->d_stamps})
     if (p) hipFree(p);
-  for (hipEvent_t e : {b->fork, b->join, b->ev0, b->ev1})
+  for (hipEvent_t e : {b->fork, b->join, b->ev0, b->ev1, b->fork7, b->join7})
     if (e) hipEventDestroy(e);
-  for (hipStream_t st : {b->side, b->master})
+  for (hipStream_t st : {b->side, b->side7, b->master})
     if (st) hipStreamDestroy(st);
   delete b;
   return DM_OK;
