@@ -1740,24 +1740,39 @@ __device__ __forceinline__ void queue_tiles(T *s, T *a, const BatchPass &bp, uin
   }
 }
 
-// The items [i0, i1) of one round, grid-stride over workgroups.  M = 6:
-// the round's 6-digit tile items and its INIT and PROBS items; M = 7: its
-// 7-digit tile items (FP32).
-template <typename T, int M>
-__global__ __launch_bounds__(QueueShape<M>::threads) __attribute__((amdgpu_waves_per_eu(M == kQueueTile7 ? 4 : 1))) void k_batch_round(
-    const BatchPass *__restrict__ passes, const BatchItem *__restrict__ items, uint32_t i0, uint32_t i1,
-    const TileOp *__restrict__ tops, const T *__restrict__ prm, T *__restrict__ arena, double *__restrict__ probs,
-    uint64_t *__restrict__ stamps) {
-  __shared__ __align__(32) unsigned char raw[QueueShape<M>::lds];
+// The 7-digit tile items [i0, i0 + gridDim.x) of one round (FP32): one
+// workgroup per item of one tile, without the round kernel's loops (with
+// them the kernel spilled at its 128 registers).
+__global__ __launch_bounds__(kQueueThreads7) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_tile7(
+    const BatchPass *__restrict__ passes, const BatchItem *__restrict__ items, uint32_t i0,
+    const TileOp *__restrict__ tops, const float *__restrict__ prm, float *__restrict__ arena) {
+  __shared__ __align__(32) float s[1 << (2 * kQueueTile7)];
+  const BatchItem it = items[i0 + blockIdx.x];
+  const BatchPass &bp = passes[it.pass];
+  float *a = arena + bp.state_off;
+  if (bp.geo.ident) queue_tiles<float, false, kQueueTile7>(s, a, bp, it.tile0, 1, tops, prm);
+  else queue_tiles<float, true, kQueueTile7>(s, a, bp, it.tile0, 1, tops, prm);
+}
+
+// The items [i0, i1) of one round, grid-stride over workgroups: its
+// 6-digit tile items and its INIT and PROBS items.
+template <typename T>
+__global__ __launch_bounds__(kQueueThreads) void k_batch_round(const BatchPass *__restrict__ passes,
+                                                               const BatchItem *__restrict__ items, uint32_t i0,
+                                                               uint32_t i1, const TileOp *__restrict__ tops,
+                                                               const T *__restrict__ prm, T *__restrict__ arena,
+                                                               double *__restrict__ probs,
+                                                               uint64_t *__restrict__ stamps) {
+  __shared__ __align__(32) unsigned char raw[kQueueLdsBytes];
   const int tid = threadIdx.x;
   for (uint32_t t = i0 + blockIdx.x; t < i1; t += gridDim.x) {
     const BatchItem it = items[t];
     const BatchPass &bp = passes[it.pass];
     T *a = arena + bp.state_off;
-    if (M == kQueueTile7 || bp.kind == BP_TILE) {
+    if (bp.kind == BP_TILE) {
       T *s = reinterpret_cast<T *>(raw);
-      if (bp.geo.ident) queue_tiles<T, false, M>(s, a, bp, it.tile0, it.ntiles, tops, prm);
-      else queue_tiles<T, true, M>(s, a, bp, it.tile0, it.ntiles, tops, prm);
+      if (bp.geo.ident) queue_tiles<T, false, kQueueTile>(s, a, bp, it.tile0, it.ntiles, tops, prm);
+      else queue_tiles<T, true, kQueueTile>(s, a, bp, it.tile0, it.ntiles, tops, prm);
     } else if (bp.kind == BP_INIT) {
       if (tid == 0 && it.tile0 == 0) stamp_now(&stamps[2 * bp.circ]);
       const T amp = (T)ldexp(1.0, -bp.n);
@@ -3133,8 +3148,7 @@ struct dm_batch {
   std::vector<uint32_t> round_split;
   uint32_t nitems = 0;
   int64_t npasses = 0;
-  int queue_grid = 0;   // resident workgroups of k_batch_round<T, 6> on the device
-  int queue_grid7 = 0;  // and of k_batch_round<float, 7>
+  int queue_grid = 0;  // resident workgroups of k_batch_round on the device
   hipStream_t master = nullptr, side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr, ev0 = nullptr, ev1 = nullptr;
   // a round's 7-digit launch runs beside its 6-digit launch (FP32)
@@ -3328,7 +3342,7 @@ int enqueue_batch(dm_batch *b) {
     }
     if (i1 > i0) {
       const int grid = (int)std::min<uint32_t>(i1 - i0, (uint32_t)b->queue_grid);
-      hipLaunchKernelGGL((k_batch_round<T, kQueueTile>), dim3(grid), dim3(kQueueThreads), 0, b->master,
+      hipLaunchKernelGGL((k_batch_round<T>), dim3(grid), dim3(kQueueThreads), 0, b->master,
                          static_cast<const BatchPass *>(b->d_passes), static_cast<const BatchItem *>(b->d_items), i0,
                          i1, static_cast<const TileOp *>(b->d_qtops), static_cast<const T *>(b->d_qvals),
                          static_cast<T *>(b->d_arena), b->d_probs, b->d_stamps + 2 * b->small.size());
@@ -3336,13 +3350,11 @@ int enqueue_batch(dm_batch *b) {
     }
     if constexpr (sizeof(T) == 4) {
       if (i2 > i1) {
-        const int grid = (int)std::min<uint32_t>(i2 - i1, (uint32_t)b->queue_grid7);
-        hipLaunchKernelGGL((k_batch_round<float, kQueueTile7>), dim3(grid), dim3(kQueueThreads7), 0,
-                           both ? b->side7 : b->master, static_cast<const BatchPass *>(b->d_passes),
-                           static_cast<const BatchItem *>(b->d_items), i1, i2,
-                           static_cast<const TileOp *>(b->d_qtops), static_cast<const T *>(b->d_qvals),
-                           static_cast<T *>(b->d_arena), b->d_probs, b->d_stamps + 2 * b->small.size());
-        if (int rc = last_launch("k_batch_round")) return rc;
+        hipLaunchKernelGGL(k_batch_tile7, dim3(i2 - i1), dim3(kQueueThreads7), 0, both ? b->side7 : b->master,
+                           static_cast<const BatchPass *>(b->d_passes), static_cast<const BatchItem *>(b->d_items), i1,
+                           static_cast<const TileOp *>(b->d_qtops), static_cast<const float *>(b->d_qvals),
+                           static_cast<float *>(b->d_arena));
+        if (int rc = last_launch("k_batch_tile7")) return rc;
       }
     }
     if (both) {
@@ -3489,7 +3501,9 @@ static int batch_create_t(dm_batch *b, const int64_t *op_start, const dm_op *ops
           if (is7 != (seven == 1)) continue;
           const uint64_t units =
               bp.kind == BP_PROBS ? 1 : pow4(bp.n - (bp.kind == BP_TILE ? bp.geo.m : kQueueTile));
-          const uint64_t per = std::max<uint64_t>(1, (units + items_per_pass - 1) / items_per_pass);
+          // (7-digit passes: one tile per item, one workgroup each)
+          const uint64_t per =
+              is7 ? 1 : std::max<uint64_t>(1, (units + items_per_pass - 1) / items_per_pass);
           for (uint64_t t0 = 0; t0 < units; t0 += per) {
             BatchItem it;
             it.pass = (uint32_t)pi;
@@ -3540,19 +3554,10 @@ static int batch_create_t(dm_batch *b, const int64_t *op_start, const dm_op *ops
     if (int rc = upload(&b->d_qtops, qtops.data(), qtops.size() * sizeof(TileOp))) return rc;
     if (int rc = upload(&b->d_qvals, tv.data(), tv.size() * sizeof(T))) return rc;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_round<T, kQueueTile>, kQueueThreads, 0) !=
-            hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_round<T>, kQueueThreads, 0) != hipSuccess ||
         per_cu < 1)
       per_cu = 1;
     b->queue_grid = cu * per_cu;
-    if constexpr (sizeof(T) == 4) {
-      int per7 = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per7, k_batch_round<float, kQueueTile7>, kQueueThreads7, 0) !=
-              hipSuccess ||
-          per7 < 1)
-        per7 = 1;
-      b->queue_grid7 = cu * per7;
-    }
     for (size_t r = 0; r + 1 < b->round_start.size(); ++r)
       b->launches += (b->round_split[r] > b->round_start[r] ? 1 : 0) + (b->round_start[r + 1] > b->round_split[r] ? 1 : 0);
   }
