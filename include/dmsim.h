@@ -225,11 +225,12 @@ int dm_mlp_train(float *params, int n_features, int hidden, const float *X, cons
  * 6-digit tile pass (DM_OP_FUSED with tile size 6, DM_OP_1Q, DM_OP_CX[_SCALED],
  * DM_OP_2Q, DM_OP_MEMORY_NOISE; memory noise as 1-qubit channels, 6 qubits
  * per pass; in FP32 also DM_OP_FUSED with tile size 7 holding single-qubit
- * and CX operations), each pass is split into at most items_per_pass items
- * of consecutive tiles, and round r (pass r of every circuit, plus the
- * |0..0> initialisation first and the probabilities last) is one
- * grid-stride launch over its items, or two when it has 7-digit passes.  A
- * run is at most 7 LDS-tier launches plus one or two launches per round
+ * and CX operations).  Round r holds pass r of every circuit, plus the
+ * |0..0> initialisation first and the probabilities last: its
+ * initialisation and probability work (split into at most items_per_pass
+ * items per pass) is one grid-stride launch, and its 6-digit and 7-digit
+ * tile passes one launch each with a workgroup per tile, the three side by
+ * side.  A run is at most 7 LDS-tier launches plus three launches per round
  * (the longest circuit's passes + 2 rounds); dm_batch_run with use_graph != 0
  * captures them into one hipGraph on first use and replays it.  Per-circuit
  * times (circuit_us) are GPU clock stamps from a circuit's initialisation to

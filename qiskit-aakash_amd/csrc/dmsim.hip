@@ -1740,22 +1740,30 @@ __device__ __forceinline__ void queue_tiles(T *s, T *a, const BatchPass &bp, uin
   }
 }
 
-// The 7-digit tile items [i0, i0 + gridDim.x) of one round (FP32): one
-// workgroup per item of one tile, without the round kernel's loops (with
-// them the kernel spilled at its 128 registers).
-__global__ __launch_bounds__(kQueueThreads7) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_tile7(
-    const BatchPass *__restrict__ passes, const BatchItem *__restrict__ items, uint32_t i0,
-    const TileOp *__restrict__ tops, const float *__restrict__ prm, float *__restrict__ arena) {
-  __shared__ __align__(32) float s[1 << (2 * kQueueTile7)];
+// The M-digit tile items [i0, i0 + gridDim.x) of one round: one workgroup
+// per item of one tile, without the round kernel's loops, so the registers
+// are those of the one-tile kernel (in the grid-stride round kernel the tile
+// code sat at the 106-SGPR limit and the 7-digit form spilled to scratch).
+// Waves per SIMD as the one-tile kernel: 7-digit FP32 tiles 4 (the two
+// 64 KB workgroups per CU), 6-digit FP32 tiles 6.
+template <typename T, int M>
+__global__ __launch_bounds__(QueueShape<M>::threads) __attribute__((amdgpu_waves_per_eu(
+    sizeof(T) == 4 ? (M == kQueueTile7 ? 4 : 6) : 1))) void k_batch_tile(const BatchPass *__restrict__ passes,
+                                                                          const BatchItem *__restrict__ items,
+                                                                          uint32_t i0,
+                                                                          const TileOp *__restrict__ tops,
+                                                                          const T *__restrict__ prm,
+                                                                          T *__restrict__ arena) {
+  __shared__ __align__(32) T s[1 << (2 * M)];
   const BatchItem it = items[i0 + blockIdx.x];
   const BatchPass &bp = passes[it.pass];
-  float *a = arena + bp.state_off;
-  if (bp.geo.ident) queue_tiles<float, false, kQueueTile7>(s, a, bp, it.tile0, 1, tops, prm);
-  else queue_tiles<float, true, kQueueTile7>(s, a, bp, it.tile0, 1, tops, prm);
+  T *a = arena + bp.state_off;
+  if (bp.geo.ident) queue_tiles<T, false, M>(s, a, bp, it.tile0, 1, tops, prm);
+  else queue_tiles<T, true, M>(s, a, bp, it.tile0, 1, tops, prm);
 }
 
-// The items [i0, i1) of one round, grid-stride over workgroups: its
-// 6-digit tile items and its INIT and PROBS items.
+// The INIT and PROBS items [i0, i1) of one round, grid-stride over
+// workgroups.
 template <typename T>
 __global__ __launch_bounds__(kQueueThreads) void k_batch_round(const BatchPass *__restrict__ passes,
                                                                const BatchItem *__restrict__ items, uint32_t i0,
@@ -1769,11 +1777,7 @@ __global__ __launch_bounds__(kQueueThreads) void k_batch_round(const BatchPass *
     const BatchItem it = items[t];
     const BatchPass &bp = passes[it.pass];
     T *a = arena + bp.state_off;
-    if (bp.kind == BP_TILE) {
-      T *s = reinterpret_cast<T *>(raw);
-      if (bp.geo.ident) queue_tiles<T, false, kQueueTile>(s, a, bp, it.tile0, it.ntiles, tops, prm);
-      else queue_tiles<T, true, kQueueTile>(s, a, bp, it.tile0, it.ntiles, tops, prm);
-    } else if (bp.kind == BP_INIT) {
+    if (bp.kind == BP_INIT) {
       if (tid == 0 && it.tile0 == 0) stamp_now(&stamps[2 * bp.circ]);
       const T amp = (T)ldexp(1.0, -bp.n);
       const uint64_t c0 = (uint64_t)it.tile0 << (2 * kQueueTile);
@@ -3144,16 +3148,18 @@ struct dm_batch {
   std::vector<int> large;
   void *d_arena = nullptr, *d_passes = nullptr, *d_items = nullptr, *d_qtops = nullptr, *d_qvals = nullptr;
   std::vector<uint32_t> round_start;  // items of round r: [round_start[r], round_start[r + 1])
-  // the 7-digit tile items of round r: [round_split[r], round_start[r + 1])
-  std::vector<uint32_t> round_split;
+  // round r: INIT and PROBS items [round_start[r], round_split6[r]), 6-digit
+  // tile items [round_split6[r], round_split[r]) and 7-digit tile items
+  // [round_split[r], round_start[r + 1]), one tile per tile item
+  std::vector<uint32_t> round_split6, round_split;
   uint32_t nitems = 0;
   int64_t npasses = 0;
   int queue_grid = 0;  // resident workgroups of k_batch_round on the device
   hipStream_t master = nullptr, side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr, ev0 = nullptr, ev1 = nullptr;
-  // a round's 7-digit launch runs beside its 6-digit launch (FP32)
-  hipStream_t side7 = nullptr;
-  hipEvent_t fork7 = nullptr, join7 = nullptr;
+  // a round's tile launches run beside its INIT / PROBS launch
+  hipStream_t side6 = nullptr, side7 = nullptr;
+  hipEvent_t forkr = nullptr, join6 = nullptr, join7 = nullptr;
   double *d_probs = nullptr;
   uint64_t *d_stamps = nullptr;  // start / end clock per circuit: LDS tier first, then the HBM tier
   hipGraph_t graph = nullptr;
@@ -3329,37 +3335,46 @@ int enqueue_batch(dm_batch *b) {
                        static_cast<const T *>(b->d_vals), b->d_probs, b->d_stamps + 2 * (size_t)lo);
     if (int rc = last_launch("k_circuit_lds")) return rc;
   }
-  // HBM tier on the master stream: one launch per round, in order; a
-  // round with 7-digit tile items launches them on side7 beside its
-  // 6-digit launch (distinct circuits: independent items), joined before
-  // the next round
+  // HBM tier: the rounds in order on the master stream.  A round's INIT /
+  // PROBS items (master), 6-digit tile items (side6) and 7-digit tile items
+  // (side7, FP32) belong to distinct circuits: the three launches run side
+  // by side, forked from and joined back into master.
   for (size_t r = 0; r + 1 < b->round_start.size(); ++r) {
-    const uint32_t i0 = b->round_start[r], i1 = b->round_split[r], i2 = b->round_start[r + 1];
-    const bool both = sizeof(T) == 4 && i1 > i0 && i2 > i1;
-    if (both) {
-      HIP_TRY(hipEventRecord(b->fork7, b->master));
-      HIP_TRY(hipStreamWaitEvent(b->side7, b->fork7, 0));
+    const uint32_t i0 = b->round_start[r], i6 = b->round_split6[r], i7 = b->round_split[r],
+                   i1 = b->round_start[r + 1];
+    const bool t6 = i7 > i6, t7 = sizeof(T) == 4 && i1 > i7;
+    if (t6 || t7) {
+      HIP_TRY(hipEventRecord(b->forkr, b->master));
+      if (t6) HIP_TRY(hipStreamWaitEvent(b->side6, b->forkr, 0));
+      if (t7) HIP_TRY(hipStreamWaitEvent(b->side7, b->forkr, 0));
     }
-    if (i1 > i0) {
-      const int grid = (int)std::min<uint32_t>(i1 - i0, (uint32_t)b->queue_grid);
+    if (i6 > i0) {
+      const int grid = (int)std::min<uint32_t>(i6 - i0, (uint32_t)b->queue_grid);
       hipLaunchKernelGGL((k_batch_round<T>), dim3(grid), dim3(kQueueThreads), 0, b->master,
                          static_cast<const BatchPass *>(b->d_passes), static_cast<const BatchItem *>(b->d_items), i0,
-                         i1, static_cast<const TileOp *>(b->d_qtops), static_cast<const T *>(b->d_qvals),
+                         i6, static_cast<const TileOp *>(b->d_qtops), static_cast<const T *>(b->d_qvals),
                          static_cast<T *>(b->d_arena), b->d_probs, b->d_stamps + 2 * b->small.size());
       if (int rc = last_launch("k_batch_round")) return rc;
     }
-    if constexpr (sizeof(T) == 4) {
-      if (i2 > i1) {
-        hipLaunchKernelGGL(k_batch_tile7, dim3(i2 - i1), dim3(kQueueThreads7), 0, both ? b->side7 : b->master,
-                           static_cast<const BatchPass *>(b->d_passes), static_cast<const BatchItem *>(b->d_items), i1,
-                           static_cast<const TileOp *>(b->d_qtops), static_cast<const float *>(b->d_qvals),
-                           static_cast<float *>(b->d_arena));
-        if (int rc = last_launch("k_batch_tile7")) return rc;
-      }
+    if (t6) {
+      hipLaunchKernelGGL((k_batch_tile<T, kQueueTile>), dim3(i7 - i6), dim3(kQueueThreads), 0, b->side6,
+                         static_cast<const BatchPass *>(b->d_passes), static_cast<const BatchItem *>(b->d_items), i6,
+                         static_cast<const TileOp *>(b->d_qtops), static_cast<const T *>(b->d_qvals),
+                         static_cast<T *>(b->d_arena));
+      if (int rc = last_launch("k_batch_tile")) return rc;
+      HIP_TRY(hipEventRecord(b->join6, b->side6));
+      HIP_TRY(hipStreamWaitEvent(b->master, b->join6, 0));
     }
-    if (both) {
-      HIP_TRY(hipEventRecord(b->join7, b->side7));
-      HIP_TRY(hipStreamWaitEvent(b->master, b->join7, 0));
+    if constexpr (sizeof(T) == 4) {
+      if (t7) {
+        hipLaunchKernelGGL((k_batch_tile<float, kQueueTile7>), dim3(i1 - i7), dim3(kQueueThreads7), 0, b->side7,
+                           static_cast<const BatchPass *>(b->d_passes), static_cast<const BatchItem *>(b->d_items),
+                           i7, static_cast<const TileOp *>(b->d_qtops), static_cast<const float *>(b->d_qvals),
+                           static_cast<float *>(b->d_arena));
+        if (int rc = last_launch("k_batch_tile")) return rc;
+        HIP_TRY(hipEventRecord(b->join7, b->side7));
+        HIP_TRY(hipStreamWaitEvent(b->master, b->join7, 0));
+      }
     }
   }
   HIP_TRY(hipEventRecord(b->join, b->side));
@@ -3490,20 +3505,21 @@ static int batch_create_t(dm_batch *b, const int64_t *op_start, const dm_op *ops
     for (auto &ids : chain) rounds = std::max(rounds, ids.size());
     for (size_t r = 0; r < rounds; ++r) {
       b->round_start.push_back((uint32_t)items.size());
-      // the round's 6-digit tile, INIT and PROBS items, then its 7-digit ones
-      for (int seven = 0; seven < 2; ++seven) {
-        if (seven) b->round_split.push_back((uint32_t)items.size());
+      // the round's INIT and PROBS items, then its 6-digit tile items, then
+      // its 7-digit ones (tile items: one tile each)
+      for (int group = 0; group < 3; ++group) {
+        if (group == 1) b->round_split6.push_back((uint32_t)items.size());
+        if (group == 2) b->round_split.push_back((uint32_t)items.size());
         for (int j : order) {
           if (r >= chain[j].size()) continue;
           const int pi = chain[j][r];
           const BatchPass &bp = passes[pi];
-          const bool is7 = bp.kind == BP_TILE && bp.geo.m == kQueueTile7;
-          if (is7 != (seven == 1)) continue;
+          const int g = bp.kind != BP_TILE ? 0 : bp.geo.m == kQueueTile7 ? 2 : 1;
+          if (g != group) continue;
           const uint64_t units =
               bp.kind == BP_PROBS ? 1 : pow4(bp.n - (bp.kind == BP_TILE ? bp.geo.m : kQueueTile));
-          // (7-digit passes: one tile per item, one workgroup each)
           const uint64_t per =
-              is7 ? 1 : std::max<uint64_t>(1, (units + items_per_pass - 1) / items_per_pass);
+              g ? 1 : std::max<uint64_t>(1, (units + items_per_pass - 1) / items_per_pass);
           for (uint64_t t0 = 0; t0 < units; t0 += per) {
             BatchItem it;
             it.pass = (uint32_t)pi;
@@ -3524,11 +3540,11 @@ static int batch_create_t(dm_batch *b, const int64_t *op_start, const dm_op *ops
   HIP_TRY(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
   HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
-  if (b->dtype == DM_FP32) {
-    HIP_TRY(hipStreamCreateWithFlags(&b->side7, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreateWithFlags(&b->fork7, hipEventDisableTiming));
-    HIP_TRY(hipEventCreateWithFlags(&b->join7, hipEventDisableTiming));
-  }
+  HIP_TRY(hipStreamCreateWithFlags(&b->side6, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&b->side7, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&b->forkr, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&b->join6, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&b->join7, hipEventDisableTiming));
   HIP_TRY(hipEventCreate(&b->ev0));
   HIP_TRY(hipEventCreate(&b->ev1));
   auto upload = [](void **dst, const void *src, size_t bytes) -> int {
@@ -3559,7 +3575,9 @@ static int batch_create_t(dm_batch *b, const int64_t *op_start, const dm_op *ops
       per_cu = 1;
     b->queue_grid = cu * per_cu;
     for (size_t r = 0; r + 1 < b->round_start.size(); ++r)
-      b->launches += (b->round_split[r] > b->round_start[r] ? 1 : 0) + (b->round_start[r + 1] > b->round_split[r] ? 1 : 0);
+      b->launches += (b->round_split6[r] > b->round_start[r] ? 1 : 0) +
+                     (b->round_split[r] > b->round_split6[r] ? 1 : 0) +
+                     (b->round_start[r + 1] > b->round_split[r] ? 1 : 0);
   }
   HIP_TRY(hipMalloc(&b->d_probs, std::max<int64_t>(1, b->nprobs) * sizeof(double)));
   HIP_TRY(hipMalloc(&b->d_stamps, 2 * (size_t)b->ncirc * sizeof(uint64_t)));
@@ -3931,15 +3949,16 @@ int dm_batch_destroy(dm_batch *b) {
   if (!b) return DM_OK;
   if (b->master) hipStreamSynchronize(b->master);
   if (b->side) hipStreamSynchronize(b->side);
+  if (b->side6) hipStreamSynchronize(b->side6);
   if (b->side7) hipStreamSynchronize(b->side7);
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
   for (void *p : {b->d_circs, b->d_tops, b->d_vals, b->d_arena, b->d_passes, b->d_items, b->d_qtops, b->d_qvals,
                   (void *)b->d_probs, (void *)b->d_stamps})
     if (p) hipFree(p);
-  for (hipEvent_t e : {b->fork, b->join, b->ev0, b->ev1, b->fork7, b->join7})
+  for (hipEvent_t e : {b->fork, b->join, b->ev0, b->ev1, b->forkr, b->join6, b->join7})
     if (e) hipEventDestroy(e);
-  for (hipStream_t st : {b->side, b->side7, b->master})
+  for (hipStream_t st : {b->side, b->side6, b->side7, b->master})
     if (st) hipStreamDestroy(st);
   delete b;
   return DM_OK;

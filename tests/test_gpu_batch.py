@@ -57,19 +57,19 @@ def pass_tiles(prog):
 
 def round_launches(progs):
     """Launches of one batch run: round r holds INIT (r = 0), pass r - 1 or
-    PROBS of every circuit; one launch for its 6-digit, INIT and PROBS items
-    and one for its 7-digit items."""
+    PROBS of every circuit; one launch for its INIT and PROBS items, one for
+    its 6-digit tile items and one for its 7-digit ones."""
     tiles = [pass_tiles(p) for p in progs]
     rounds = max(len(t) for t in tiles) + 2
     n = 0
     for r in range(rounds):
-        six = seven = False
+        ends = six = seven = False
         for t in tiles:
             if r == 0 or r == len(t) + 1:
-                six = True
+                ends = True
             elif r <= len(t):
                 six, seven = six or t[r - 1] == 6, seven or t[r - 1] == 7
-        n += six + seven
+        n += ends + six + seven
     return n
 
 
