@@ -6,11 +6,12 @@ gate, dm_simulator.py:1172-1177), and finally an ensemble Z measurement of
 all qubits with polarisation retention ``depolarization_factor``.  Small
 registers (<= ``lds_max_qubits``) run whole inside one workgroup's LDS; the
 others keep their state in one HBM arena and run as fused 6-digit tile
-passes (7-digit in FP32 where 6 digits would leave contiguous runs of only
-4^2 coefficients), one launch per round (pass r of every circuit; two when
-the round has 7-digit passes), each pass split into at most
-``items_per_pass`` work items; the batch is captured once into a hipGraph
-and replayed (``dm_batch_*`` in dmsim.h).
+passes (7-digit in FP32 where 6 digits would leave contiguous runs of 4^2
+or 4^3 coefficients that 7 digits lengthen); round r (pass r of every
+circuit) is up to three launches side by side: the initialisation and
+probability work, split into at most ``items_per_pass`` work items per
+pass, and one launch per tile size with a workgroup per tile; the batch is
+captured once into a hipGraph and replayed (``dm_batch_*`` in dmsim.h).
 
 ``autotune`` picks the LDS cutoff width, the LDS workgroup size and the
 HBM tier's items per pass by timing the candidates on the batch itself; the
@@ -191,7 +192,8 @@ def _tune_key(specs, precision):
 def autotune(specs, precision='fp64', reps=2, use_cache=True, programs=None):
     """Best (lds_max_qubits, lds_threads, items_per_pass) for this batch by
     measured time: the LDS cutoff and workgroup size first (items per pass
-    256), then the items per pass at that cutoff."""
+    256), then the items per pass (which splits the initialisation) at that
+    cutoff."""
     key = _tune_key(specs, precision)
     path = os.environ.get('DMSIM_TUNE_CACHE')
     if use_cache:
