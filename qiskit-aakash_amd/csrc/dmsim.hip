@@ -1211,12 +1211,13 @@ __device__ __forceinline__ void stage_io(T *__restrict__ a, uint64_t base, int w
 // One tile per workgroup (grid = tiles, tile = block index): no prefetch of
 // a next tile, so fewer registers during the operations and more resident
 // workgroups per CU, which overlap each other's HBM and LDS phases.  Runs
-// are >= 4^3 coefficients: nontemporal loads and stores (measured faster on
-// MI355X for such runs, 6 % on 7-digit tiles).  FP32: 8 waves per SIMD (64
+// are >= 4^3 coefficients: nontemporal loads and stores (NTM; measured
+// faster on MI355X for FP32 runs, 6 % on 7-digit tiles; FP64 keeps plain
+// accesses).  FP32: 8 waves per SIMD (64
 // VGPRs), 7 for swizzled tiles (72 VGPRs: their XOR addresses cannot use
 // instruction offsets; 64 spilled); 7-digit tiles 4 waves (128 VGPRs), the
 // two 512-thread workgroups their 64 KB of LDS allow per CU.
-template <typename T, int M, bool GEN2Q, int NT, bool SWZ>
+template <typename T, int M, bool GEN2Q, int NT, bool SWZ, bool NTM = true>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? ((1 << (2 * M)) / NT >= 32 ? 4 : SWZ && NT <= 256 ? 7 : 8) : 1))) void k_tile_stream(T *__restrict__ a, TileGeom geo, const TileOp *__restrict__ ops,
                                                         int nops, const T *__restrict__ prm, int io, int last_at) {
   constexpr int E = 1 << (2 * M);
@@ -1237,12 +1238,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
   // the stage with header last_at (the last operation) writes HBM itself
   constexpr bool VEC = E / NT >= 32;
   if ((io & 3) == 3 && last_at == 0) {
-    stage_io<T, M, GEN2Q, NT, SWZ, VEC, 2>(a, base, w, lomask, geo, s, ops, 0, prm);
+    stage_io<T, M, GEN2Q, NT, SWZ, VEC, 2, NTM>(a, base, w, lomask, geo, s, ops, 0, prm);
     return;
   }
   int i0 = 0;
   if (io & 1) {
-    stage_io<T, M, GEN2Q, NT, SWZ, VEC, 0>(a, base, w, lomask, geo, s, ops, 0, prm);
+    stage_io<T, M, GEN2Q, NT, SWZ, VEC, 0, NTM>(a, base, w, lomask, geo, s, ops, 0, prm);
     i0 = 1 + ops[0].k;
   } else if constexpr (VEC) {
     // 2 workgroups per CU: every load of the tile is issued before the first
@@ -1253,7 +1254,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
 #pragma unroll
     for (int i = 0; i < QPT; ++i) {
       const int c = tid + i * NT;
-      if (c < E / 4) buf[i] = ldq<T, true>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo));
+      if (c < E / 4) buf[i] = ldq<T, NTM>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo));
     }
 #pragma unroll
     for (int i = 0; i < QPT; ++i) {
@@ -1264,14 +1265,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
 #pragma unroll
     for (int i = 0; i < QPT; ++i) {
       const int c = tid + i * NT;
-      if (c < E / 4) lds_put_quad(s, lds_swz<SWZ>(4 * c), ldq<T, true>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo)));
+      if (c < E / 4) lds_put_quad(s, lds_swz<SWZ>(4 * c), ldq<T, NTM>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo)));
     }
   }
   __syncthreads();
   const int i1 = (io & 2) ? last_at : nops;
   tile_ops<T, M, false, GEN2Q, NT, SWZ, VEC>(s, ops + i0, i1 - i0, prm);
   if (io & 2) {
-    stage_io<T, M, GEN2Q, NT, SWZ, VEC, 1>(a, base, w, lomask, geo, s, ops, last_at, prm);
+    stage_io<T, M, GEN2Q, NT, SWZ, VEC, 1, NTM>(a, base, w, lomask, geo, s, ops, last_at, prm);
   } else if constexpr (VEC) {
     Q buf[QPT];
 #pragma unroll
@@ -1282,13 +1283,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
 #pragma unroll
     for (int i = 0; i < QPT; ++i) {
       const int c = tid + i * NT;
-      if (c < E / 4) stq<T, true>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), buf[i]);
+      if (c < E / 4) stq<T, NTM>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), buf[i]);
     }
   } else {
 #pragma unroll
     for (int i = 0; i < QPT; ++i) {
       const int c = tid + i * NT;
-      if (c < E / 4) stq<T, true>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), lds_get_quad(s, lds_swz<SWZ>(4 * c)));
+      if (c < E / 4) stq<T, NTM>(a + base + tile_offset<T, M>(4 * c, w, lomask, geo), lds_get_quad(s, lds_swz<SWZ>(4 * c)));
     }
   }
 }
@@ -2818,6 +2819,9 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
   //    LDS allows 2): k_tile_stream, one tile per workgroup in tile order
   //    (the hardware overlaps the load, LDS and store phases of different
   //    workgroups);
+  //  - FP64 6-digit tiles over runs of >= 4^3: k_tile_stream with plain
+  //    accesses (110 VGPRs, 4 waves per SIMD; k_tile_prog's 146-162 leave
+  //    3);
   //  - other runs of >= 4^3 with 2 or more resident workgroups per CU:
   //    k_tile_prog with one tile per workgroup, whose loads all issue before
   //    the LDS writes; in FP32, where k_tile_prog's registers leave one
@@ -2842,6 +2846,13 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
       hipLaunchKernelGGL((k_tile_stream<T, MM, G, NTH, L>), dim3((unsigned)fp.geo.tiles), dim3(NTH), 0,          \
                          h->stream, a, fp.geo, d_ops, nops, d_vals, io, last_stage);                             \
       break;                                                                                                     \
+    }                                                                                                            \
+    if constexpr (sizeof(T) == 8 && MM == 6 && !D) {                                                             \
+      if (long_runs) {                                                                                           \
+        hipLaunchKernelGGL((k_tile_stream<T, MM, G, NTH, L, false>), dim3((unsigned)fp.geo.tiles), dim3(NTH), 0, \
+                           h->stream, a, fp.geo, d_ops, nops, d_vals, io, last_stage);                           \
+        break;                                                                                                   \
+      }                                                                                                          \
     }                                                                                                            \
     const bool full = long_runs && per_cu >= 2;                                                                  \
     const uint64_t slots = (uint64_t)h->cu_count * per_cu * (per_cu >= 2 ? 8 : 4);                              \
