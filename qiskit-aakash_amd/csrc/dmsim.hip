@@ -2810,6 +2810,12 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
   // copy where their group order suits both HBM and LDS
   int last_stage = 0;
   const int io = stage_io_flags(fp, &last_stage);
+  // FP64 7-digit one-tile passes (1024 threads): the HBM-side stages of the
+  // 16-byte form (stages on digit 0) ran slower than the staging copy
+  // (16 qubits: 17.7 against 15.0 ms), those of the other forms faster
+  int io_no_quad = io;
+  if ((io & 1) && fp.ops[0].d[0] == 0) io_no_quad &= ~1;
+  if ((io & 2) && fp.ops[last_stage].d[0] == 0) io_no_quad &= ~2;
   // one variant per feature set keeps the register footprint of the common
   // 1q + CX passes small; 7-qubit tiles (64 / 128 KB of LDS) use 512 threads
   // Grid shape, from 16-qubit sweeps on MI355X:
@@ -2821,7 +2827,10 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
   //    workgroups);
   //  - FP64 6-digit tiles over runs of >= 4^3: k_tile_stream with plain
   //    accesses (110 VGPRs, 4 waves per SIMD; k_tile_prog's 146-162 leave
-  //    3);
+  //    3); FP64 7-digit tiles over such runs (without 16x16 members): the
+  //    same with 1024 threads (113 VGPRs, 4 waves per SIMD in the one
+  //    128 KB workgroup per CU, against 2 for the persistent kernel of 512,
+  //    which spills at 1024);
   //  - other runs of >= 4^3 with 2 or more resident workgroups per CU:
   //    k_tile_prog with one tile per workgroup, whose loads all issue before
   //    the LDS writes; in FP32, where k_tile_prog's registers leave one
@@ -2847,10 +2856,10 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
                          h->stream, a, fp.geo, d_ops, nops, d_vals, io, last_stage);                             \
       break;                                                                                                     \
     }                                                                                                            \
-    if constexpr (sizeof(T) == 8 && MM == 6 && !D) {                                                             \
+    if constexpr (sizeof(T) == 8 && (MM == 6 || NTH == 1024) && !D) {                                            \
       if (long_runs) {                                                                                           \
         hipLaunchKernelGGL((k_tile_stream<T, MM, G, NTH, L, false>), dim3((unsigned)fp.geo.tiles), dim3(NTH), 0, \
-                           h->stream, a, fp.geo, d_ops, nops, d_vals, io, last_stage);                           \
+                           h->stream, a, fp.geo, d_ops, nops, d_vals, MM == 7 ? io_no_quad : io, last_stage);    \
         break;                                                                                                   \
       }                                                                                                          \
     }                                                                                                            \
@@ -2867,6 +2876,7 @@ int launch_fused(dm_state *h, const FusedPass &fp, const TileOp *d_ops, const T 
     else TILE_LAUNCH_L(MM, D, G, NTH, true);     \
   } while (0)
   if (fp.m == 7 && gen) TILE_LAUNCH(7, false, true, 512);
+  else if (fp.m == 7 && sizeof(T) == 8 && fp.geo.w >= 3) TILE_LAUNCH(7, false, false, 1024);
   else if (fp.m == 7) TILE_LAUNCH(7, false, false, 512);
   else if (fp.m == 6 && dense) TILE_LAUNCH_L(6, true, true, 256, false);
   else if (fp.m == 6 && gen) TILE_LAUNCH(6, false, true, 256);
