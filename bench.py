@@ -368,6 +368,9 @@ def batch_mode(args):
             (lmax, thr, ipp), tune = batch.autotune(specs, prec, programs=programs)
             log('  chose lds_max_qubits %d, %d threads, %d items per pass (%.1f s)' % (lmax, thr, ipp,
                                                                                      time.perf_counter() - t0))
+        t0 = time.perf_counter()
+        batch.workload_programs(specs, prec, lmax)
+        host_build_s = time.perf_counter() - t0
         b = batch.from_workload(specs, prec, lds_max_qubits=lmax, lds_threads=thr, items_per_pass=ipp,
                                 programs=programs)
         info = b.info()
@@ -383,7 +386,8 @@ def batch_mode(args):
         us = np.concatenate(us_all)
         lines[prec] = {'circuits_per_s': count * args.steps / wall, 'p50_us': float(np.percentile(us, 50)),
                        'p99_us': float(np.percentile(us, 99)), 'lds_max_qubits': lmax, 'lds_threads': thr,
-                       'items_per_pass': ipp, 'hbm_tier': info, 'autotune_ms': tune.get('trials_ms')}
+                       'items_per_pass': ipp, 'hbm_tier': info, 'autotune_ms': tune.get('trials_ms'),
+                       'host_build_s': host_build_s}
         log('%s: %s' % (prec, json.dumps(lines[prec])))
     return {'metric': 'circuits/sec, %d noisy circuits of %d-%d qubits (one hipGraph: LDS-tier launches + one '
                       'launch per round of HBM-tier passes)' % (count, lo, hi),

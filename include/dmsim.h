@@ -244,6 +244,32 @@ int dm_batch_info(const dm_batch *b, int64_t *hbm_circuits, int64_t *passes, int
 int dm_batch_run(dm_batch *b, int use_graph, double *probs_out, double *circuit_us, double *total_ms);
 int dm_batch_destroy(dm_batch *b);
 
+/* The program of one batch circuit, built on the host (no device work; no
+ * reference counterpart: the reference interprets each experiment's
+ * instructions level by level, dm_simulator.py:1005-1177, with a barrier
+ * after every gate giving one memory-noise level per gate, :1172-1177).
+ * Gate i: kind[i] = DM_INST_U1 (angles[3i] = lambda), DM_INST_U3 (angles
+ * [3i..3i+2] = theta, phi, lambda; the PTM rz(phi) ry(theta) rz(lambda),
+ * basicaertools.py:69-90) or DM_INST_CX (q0 control, q1 target); every gate
+ * is followed by the depolarising channel gate_depolarization on its qubits
+ * (none at 0) and by memory noise (decoherence, thermal, decay factors, the
+ * reference's retention factors).  fused == 0: each gate as its DM_OP_1Q /
+ * DM_OP_CX[_SCALED] record, then one DM_OP_MEMORY_NOISE.  fused != 0 (n >= 6):
+ * memory noise as one 1-qubit channel per qubit (none when decay and
+ * decoherence are 1), the greedy plan of dm_plan_fusion (blocks of up to 4
+ * qubits), and each block as one DM_OP_FUSED pass of 6 digits (FP32: 7 where
+ * 6 would leave contiguous runs of 4^2 coefficients that 7 lengthens, or
+ * runs of 4^3 that 7 makes 4^4), runs of 1-qubit PTMs on one qubit folded;
+ * single-gate blocks keep their plain records and one-qubit blocks become
+ * DM_OP_1Q.  precision (DM_FP64 / DM_FP32) is the state's.  *nops and
+ * *nvalues receive the sizes; with ops and values both NULL only the sizes
+ * are computed, else the records go to ops[0..max_ops) and values
+ * [0..max_values) (DM_ERR_ARG when they do not fit). */
+int dm_batch_circuit_program(int n, int ngates, const int32_t *kind, const int32_t *q0, const int32_t *q1,
+                             const double *angles, double decay_factor, double decoherence_factor,
+                             double thermal_factor, double gate_depolarization, int fused, int precision, dm_op *ops,
+                             int max_ops, double *values, int64_t max_values, int *nops, int64_t *nvalues);
+
 /* Host front end (no device work; plain C++ over integer arrays).
  *
  * Instruction kinds of the merge and the partition: */

@@ -781,3 +781,244 @@ int dm_cost_block_seconds(int n, int ngates, const int32_t *gate_qubits, int nbl
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batch circuit programs (no reference counterpart): the op records of one
+// circuit of a batch, built on the host from its gate list.  A circuit has a
+// barrier after every gate, so every gate is followed by one memory-noise
+// level (dm_simulator.py:1172-1177, _add_decoherence_and_amp_decay :397-425).
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// A noisy gate: a 4x4 PTM on one qubit, or a CX with optional per-pair
+// scales (the depolarising channel on both qubits).
+struct NoisyOp {
+  int nq;
+  int q[2];
+  bool scaled;
+  double R[16];  // 1 qubit: row-major 4x4 PTM; CX: 16 per-pair scales
+};
+
+void mat4_mul(const double *A, const double *B, double *C) {  // C = A B (C may alias B)
+  double t[16];
+  for (int i = 0; i < 4; ++i)
+    for (int k = 0; k < 4; ++k) {
+      double s = 0.0;
+      for (int j = 0; j < 4; ++j) s += A[4 * i + j] * B[4 * j + k];
+      t[4 * i + k] = s;
+    }
+  memcpy(C, t, sizeof(t));
+}
+
+// M <- R M, R the rotation of the component pair (k0, k1) by angle:
+// (k0, k1) -> (c k0 - s k1, c k1 + s k0) (rot_gate_dm_matrix,
+// basicaertools.py:93-126, without rotation error).
+void rotate4(int k0, int k1, double angle, double *M) {
+  double R[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  const double c = cos(angle + 0.0), s = sin(angle + 0.0);
+  R[4 * k0 + k0] = c;
+  R[4 * k0 + k1] = -s;
+  R[4 * k1 + k1] = c;
+  R[4 * k1 + k0] = s;
+  mat4_mul(R, M, M);
+}
+
+// u3(theta, phi, lambda) = rz(phi) ry(theta) rz(lambda) and u1(lambda) =
+// rz(lambda) as 4x4 PTMs (single_gate_dm_matrix, basicaertools.py:69-90);
+// component pairs rz: (X, Y), ry: (Z, X).
+void rotation_ptm(int kind, const double *a, double *M) {
+  static const double I4[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  memcpy(M, I4, sizeof(I4));
+  if (kind == DM_INST_U1) {
+    rotate4(1, 2, a[0], M);
+  } else {
+    rotate4(1, 2, a[2], M);
+    rotate4(3, 1, a[0], M);
+    rotate4(1, 2, a[1], M);
+  }
+}
+
+// Tile digits of a batched pass over the block's positions: 6, or in FP32
+// 7 where 6 would leave runs of only 4^2 coefficients that 7 lengthens, or
+// runs of 4^3 that 7 makes 4^4.
+int batch_tile_digits(int n, uint32_t positions, int nq, bool fp32) {
+  if (fp32 && n >= 7 && nq > 1) {
+    const int w6 = run_digits(n, positions, 6), w7 = run_digits(n, positions, 7);
+    if ((w6 < 3 && w7 > w6) || (w6 == 3 && w7 == 4)) return 7;
+  }
+  return 6;
+}
+
+struct ProgramOut {
+  std::vector<dm_op> ops;
+  std::vector<double> vals;
+  void add(int kind, int k, const int *qs, int nq, const double *v, int nv) {
+    dm_op o;
+    memset(&o, 0, sizeof(o));
+    o.kind = kind;
+    o.k = k;
+    for (int j = 0; j < nq; ++j) o.qubits[j] = qs[j];
+    o.param = (int64_t)vals.size();
+    ops.push_back(o);
+    vals.insert(vals.end(), v, v + nv);
+  }
+  void plain(const NoisyOp &g) {  // a gate on its own kernel
+    if (g.nq == 1)
+      add(DM_OP_1Q, 1, g.q, 1, g.R + 4, 12);
+    else if (g.scaled)
+      add(DM_OP_CX_SCALED, 2, g.q, 2, g.R, 16);
+    else
+      add(DM_OP_CX, 2, g.q, 2, nullptr, 0);
+  }
+};
+
+}  // namespace
+
+extern "C" int dm_batch_circuit_program(int n, int ngates, const int32_t *kind, const int32_t *q0,
+                                        const int32_t *q1, const double *angles, double decay_factor,
+                                        double decoherence_factor, double thermal_factor,
+                                        double gate_depolarization, int fused, int precision, dm_op *ops,
+                                        int max_ops, double *values, int64_t max_values, int *nops,
+                                        int64_t *nvalues) {
+  if (n < 1 || n > 12 || ngates < 0 || (ngates > 0 && (!kind || !q0 || !q1 || !angles)) || !nops || !nvalues ||
+      max_ops < 0 || max_values < 0 || (precision != DM_FP64 && precision != DM_FP32))
+    return fail(DM_ERR_ARG, "dm_batch_circuit_program: bad arguments");
+  if (fused && n < 6) return fail(DM_ERR_ARG, "dm_batch_circuit_program: fused programs need n >= 6 (n=%d)", n);
+  const double p = gate_depolarization, lam = 1.0 - p;
+  std::vector<NoisyOp> g;
+  g.reserve((size_t)ngates * (fused ? n + 1 : 1));
+  ProgramOut out;
+  // memory noise: x,y <- sqrt(g) f (x,y); z <- g z + (1-g)(2p-1) a_I
+  double mem[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  const double od = sqrt(decay_factor) * decoherence_factor;
+  mem[5] = mem[10] = od;
+  mem[15] = decay_factor;
+  mem[12] = (1 - decay_factor) * (2 * thermal_factor - 1);
+  const bool mem_identity = decay_factor == 1.0 && decoherence_factor == 1.0;
+  const double level[3] = {decoherence_factor, thermal_factor, decay_factor};
+  for (int i = 0; i < ngates; ++i) {
+    NoisyOp o;
+    memset(&o, 0, sizeof(o));
+    if (kind[i] == DM_INST_U1 || kind[i] == DM_INST_U3) {
+      if (q0[i] < 0 || q0[i] >= n) return fail(DM_ERR_ARG, "dm_batch_circuit_program: gate %d has bad qubits", i);
+      o.nq = 1;
+      o.q[0] = q0[i];
+      rotation_ptm(kind[i], angles + 3 * i, o.R);
+      if (p != 0.0)  // depolarising channel after the gate: X, Y, Z rows times 1-p
+        for (int e = 4; e < 16; ++e) o.R[e] *= lam;
+    } else if (kind[i] == DM_INST_CX) {
+      if (q0[i] < 0 || q0[i] >= n || q1[i] < 0 || q1[i] >= n || q0[i] == q1[i])
+        return fail(DM_ERR_ARG, "dm_batch_circuit_program: gate %d has bad qubits", i);
+      o.nq = 2;
+      o.q[0] = q0[i];
+      o.q[1] = q1[i];
+      o.scaled = p != 0.0;
+      if (o.scaled)  // (1-p)^(number of non-identity components of the pair)
+        for (int a = 0; a < 4; ++a)
+          for (int b = 0; b < 4; ++b) o.R[4 * a + b] = pow(lam, (double)((a != 0) + (b != 0)));
+    } else {
+      return fail(DM_ERR_ARG, "dm_batch_circuit_program: gate %d is not u1, u3 or cx (kind %d)", i, kind[i]);
+    }
+    if (!fused) {
+      out.plain(o);
+      out.add(DM_OP_MEMORY_NOISE, 0, nullptr, 0, level, 3);
+      continue;
+    }
+    g.push_back(o);
+    if (!mem_identity)
+      for (int q = 0; q < n; ++q) {
+        NoisyOp m;
+        memset(&m, 0, sizeof(m));
+        m.nq = 1;
+        m.q[0] = q;
+        memcpy(m.R, mem, sizeof(mem));
+        g.push_back(m);
+      }
+  }
+  if (fused && !g.empty()) {
+    const int ng = (int)g.size();
+    std::vector<int32_t> gq(2 * (size_t)ng), members(ng), bstart(ng + 1), bq(2 * (size_t)ng), qstart(ng + 1);
+    for (int i = 0; i < ng; ++i) {
+      gq[2 * i] = g[i].q[0];
+      gq[2 * i + 1] = g[i].nq == 2 ? g[i].q[1] : -1;
+    }
+    int nb = 0;
+    if (int rc = dm_plan_fusion(n, ng, gq.data(), DM_PLAN_GREEDY, 4, 0, nullptr, 0, nullptr, precision,
+                                members.data(), bstart.data(), bq.data(), qstart.data(), &nb))
+      return rc;
+    std::vector<double> rec;
+    for (int b = 0; b < nb; ++b) {
+      const int m0 = bstart[b], m1 = bstart[b + 1];
+      const int *qs = bq.data() + qstart[b];
+      const int nq = qstart[b + 1] - qstart[b];
+      if (m1 - m0 == 1) {
+        out.plain(g[members[m0]]);
+        continue;
+      }
+      if (nq == 1) {  // a run on one qubit: the product of its PTMs
+        double M[16];
+        memcpy(M, g[members[m0]].R, sizeof(M));
+        for (int j = m0 + 1; j < m1; ++j) mat4_mul(g[members[j]].R, M, M);
+        out.add(DM_OP_1Q, 1, qs, 1, M + 4, 12);
+        continue;
+      }
+      uint32_t pos = 0;
+      for (int j = 0; j < nq; ++j) pos |= 1u << (n - 1 - qs[j]);
+      const int tile = batch_tile_digits(n, pos, nq, precision == DM_FP32);
+      // sub-operations: each run of 1-qubit PTMs on one qubit folded into one,
+      // emitted before the next CX on its qubit (or at the end, by qubit)
+      std::vector<std::pair<int, std::vector<double>>> subs;  // (-1 - qubit) for 1q, else member index
+      double pend[kMaxPos][16];
+      bool has[kMaxPos] = {};
+      auto flush = [&](int q) {
+        if (!has[q]) return;
+        subs.emplace_back(-1 - q, std::vector<double>(pend[q] + 4, pend[q] + 16));
+        has[q] = false;
+      };
+      for (int j = m0; j < m1; ++j) {
+        const NoisyOp &o = g[members[j]];
+        if (o.nq == 1) {
+          const int q = o.q[0];
+          if (has[q]) {
+            mat4_mul(o.R, pend[q], pend[q]);
+          } else {
+            memcpy(pend[q], o.R, sizeof(o.R));
+            has[q] = true;
+          }
+          continue;
+        }
+        flush(o.q[0]);
+        flush(o.q[1]);
+        subs.emplace_back(members[j], o.scaled ? std::vector<double>(o.R, o.R + 16) : std::vector<double>());
+      }
+      for (int q = 0; q < n; ++q) flush(q);
+      rec.clear();
+      rec.push_back(tile);
+      rec.push_back(nq);
+      for (int j = 0; j < nq; ++j) rec.push_back(qs[j]);
+      rec.push_back((double)subs.size());
+      for (const auto &sv : subs) {
+        if (sv.first < 0) {
+          const double h[7] = {1, 1, (double)(-1 - sv.first), 0, 0, 0, 12};
+          rec.insert(rec.end(), h, h + 7);
+        } else {
+          const NoisyOp &o = g[sv.first];
+          const double h[7] = {2, 2, (double)o.q[0], (double)o.q[1], 0, 0, (double)sv.second.size()};
+          rec.insert(rec.end(), h, h + 7);
+        }
+        rec.insert(rec.end(), sv.second.begin(), sv.second.end());
+      }
+      out.add(DM_OP_FUSED, 0, nullptr, 0, rec.data(), (int)rec.size());
+    }
+  }
+  *nops = (int)out.ops.size();
+  *nvalues = (int64_t)out.vals.size();
+  if (!ops && !values) return DM_OK;  // size query
+  if (!ops || !values || *nops > max_ops || *nvalues > max_values)
+    return fail(DM_ERR_ARG, "dm_batch_circuit_program: output holds %d ops / %lld values, need %d / %lld", max_ops,
+                (long long)max_values, *nops, (long long)*nvalues);
+  memcpy(ops, out.ops.data(), out.ops.size() * sizeof(dm_op));
+  memcpy(values, out.vals.data(), out.vals.size() * sizeof(double));
+  return DM_OK;
+}

@@ -40,6 +40,17 @@ LDS_THREAD_CHOICES = (256, 512, 1024)
 ITEMS_PER_PASS_CHOICES = (64, 256, 1024)
 
 
+def native_program(n, gates, decay_factor=1.0, decoherence_factor=1.0, thermal_factor=1.0,
+                   gate_depolarization=0.0, fused=False, precision='fp64'):
+    """The program of ``circuit_program`` (``fused`` false) or
+    ``fused_circuit_program`` (``fused`` true), built by the library on the
+    host (dm_batch_circuit_program); the two Python builders below are its
+    cross-check."""
+    ops, nops, vals = native.batch_circuit_program(n, gates, decay_factor, decoherence_factor, thermal_factor,
+                                                   gate_depolarization, fused, precision)
+    return Program.from_compiled(n, ops, nops, vals)
+
+
 def circuit_program(n, gates, decay_factor=1.0, decoherence_factor=1.0, thermal_factor=1.0,
                     gate_depolarization=0.0):
     """Program of one circuit: each gate then one memory-noise level."""
@@ -103,19 +114,17 @@ class Batch:
         """``circuits``: list of (n, Program, meas_err)."""
         self.lib = native.load()
         self.ns = [int(c[0]) for c in circuits]
-        ops, params, starts = [], [], [0]
+        recs, params, starts = [], [], [0]
         off = 0
         for n, prog, _ in circuits:
             arr, pv = prog.compile()
-            for i in range(len(prog)):
-                o = native.DmOp()
-                ctypes.pointer(o)[0] = arr[i]
-                o.param = arr[i].param + off
-                ops.append(o)
+            rec = np.frombuffer(arr, dtype=native.DMOP_DTYPE, count=len(prog)).copy()
+            rec['param'] += off
+            recs.append(rec)
             params.append(pv)
             off += pv.size
-            starts.append(len(ops))
-        self._ops = (native.DmOp * max(1, len(ops)))(*ops)
+            starts.append(starts[-1] + len(prog))
+        self._ops = np.concatenate(recs) if starts[-1] else np.zeros(1, dtype=native.DMOP_DTYPE)
         self._params = np.ascontiguousarray(np.concatenate(params) if params else np.zeros(1))
         self._starts = np.asarray(starts, dtype=np.int64)
         self._n = np.asarray(self.ns, dtype=np.int32)
@@ -124,7 +133,7 @@ class Batch:
         native.check(self.lib.dm_batch_create(
             native.FP32 if precision == 'fp32' else native.FP64, len(circuits),
             self._n.ctypes.data_as(native.c_int_p), self._starts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
-            ctypes.cast(self._ops, ctypes.c_void_p), self._params.ctypes.data_as(native.c_double_p),
+            ctypes.c_void_p(self._ops.ctypes.data), self._params.ctypes.data_as(native.c_double_p),
             int(off), self._err.ctypes.data_as(native.c_double_p), int(lds_max_qubits), int(lds_threads),
             int(items_per_pass), ctypes.byref(h)), 'dm_batch_create')
         self.h = h
@@ -165,23 +174,27 @@ class Batch:
 def from_workload(specs, precision='fp64', lds_max_qubits=7, programs=None, **kw):
     """Batch from dmsim.workloads.batch specs (n, gates, decay_rate, depol_rate):
     decay_factor = 1 - decay_rate, measurement retention = 1 - depol_rate.
-    Circuits above the LDS cutoff run as fused tile passes.  ``programs``, a
-    dict, keeps the built programs per (circuit, form) for later calls on the
-    same specs."""
+    Circuits above the LDS cutoff run as fused tile passes.  The programs
+    are built by the library (``native_program``).  ``programs``, a dict,
+    keeps them per (circuit, form) for later calls on the same specs."""
+    return Batch(workload_programs(specs, precision, lds_max_qubits, programs), precision,
+                 lds_max_qubits=lds_max_qubits, **kw)
+
+
+def workload_programs(specs, precision='fp64', lds_max_qubits=7, programs=None):
+    """(n, Program, measurement retention) of each spec, as ``from_workload``
+    hands them to the batch."""
     circuits = []
     for i, (n, gates, dr, pr) in enumerate(specs):
         fused = n > lds_max_qubits
         key = (i, fused, precision if fused else None)
         prog = programs.get(key) if programs is not None else None
         if prog is None:
-            if fused:
-                prog = fused_circuit_program(n, gates, decay_factor=1.0 - dr, precision=precision)
-            else:
-                prog = circuit_program(n, gates, decay_factor=1.0 - dr)
+            prog = native_program(n, gates, decay_factor=1.0 - dr, fused=fused, precision=precision)
             if programs is not None:
                 programs[key] = prog
         circuits.append((n, prog, 1.0 - pr))
-    return Batch(circuits, precision, lds_max_qubits=lds_max_qubits, **kw)
+    return circuits
 
 
 def _tune_key(specs, precision):

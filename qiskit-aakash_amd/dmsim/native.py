@@ -16,6 +16,8 @@ _lib = None
 _lock = threading.Lock()
 
 FP64, FP32 = 0, 1
+# return codes (DM_ERR_* of include/dmsim.h)
+ERR_ARG, ERR_HIP, ERR_STATE = -1, -2, -3
 
 c_double_p = ctypes.POINTER(ctypes.c_double)
 c_int_p = ctypes.POINTER(ctypes.c_int)
@@ -82,6 +84,10 @@ SIGNATURES = {
     'dm_batch_info': (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     'dm_batch_run': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_double_p, c_double_p, c_double_p]),
+    'dm_batch_circuit_program': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, c_int_p, c_int_p, c_int_p, c_double_p,
+                                                ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, c_double_p,
+                                                ctypes.c_int64, c_int_p, ctypes.POINTER(ctypes.c_int64)]),
     'dm_batch_destroy': (ctypes.c_int, [ctypes.c_void_p]),
     'dm_synchronize': (ctypes.c_int, [ctypes.c_void_p]),
     'dm_profile': (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -94,6 +100,11 @@ class DmOp(ctypes.Structure):
     _fields_ = [('kind', ctypes.c_int32), ('k', ctypes.c_int32), ('qubits', ctypes.c_int32 * 4),
                 ('param', ctypes.c_int64)]
 
+
+# the same record as a numpy structured type (views of DmOp arrays)
+DMOP_DTYPE = np.dtype({'names': ['kind', 'k', 'qubits', 'param'], 'formats': ['<i4', '<i4', ('<i4', (4,)), '<i8'],
+                       'offsets': [0, 4, 8, 24], 'itemsize': 32})
+assert ctypes.sizeof(DmOp) == DMOP_DTYPE.itemsize
 
 OP_1Q, OP_CX, OP_CX_SCALED, OP_2Q, OP_BLOCK, OP_MEMORY_NOISE, OP_DIAG, OP_FUSED = 1, 2, 3, 4, 5, 6, 7, 8
 # sub-operation kinds of a fused pass
@@ -459,6 +470,56 @@ def plan_fusion(n, gate_qubits, method, kmax, tiles, lookahead=(), params=None, 
                                 _ip(qstart), ctypes.byref(nb)), 'dm_plan_fusion')
     return [(qubits[qstart[b]:qstart[b + 1]].tolist(), members[bstart[b]:bstart[b + 1]].tolist())
             for b in range(nb.value)]
+
+
+def batch_circuit_program(n, gates, decay_factor=1.0, decoherence_factor=1.0, thermal_factor=1.0,
+                          gate_depolarization=0.0, fused=False, precision='fp64'):
+    """(DmOp array, values) of one batch circuit from dm_batch_circuit_program:
+    gates (name, qubits, params) of u1 / u2 / u3 / cx, each followed by a
+    memory-noise level; ``fused`` gives the tile-pass form."""
+    m = len(gates)
+    kind, q0, q1, ang = [], [], [], []
+    for name, qs, prm in gates:
+        q0.append(qs[0])
+        if name == 'cx':
+            kind.append(INST_CX)
+            q1.append(qs[1])
+            ang += (0.0, 0.0, 0.0)
+            continue
+        q1.append(0)
+        if name in ('u3', 'U'):
+            kind.append(INST_U3)
+            ang += prm[:3]
+        elif name == 'u2':
+            kind.append(INST_U3)
+            ang += (np.pi / 2, prm[0], prm[1])
+        elif name == 'u1':
+            kind.append(INST_U1)
+            ang += (prm[0], 0.0, 0.0)
+        else:
+            raise ValueError('batch circuits hold u1/u2/u3/cx gates, got %s' % name)
+    kind, q0, q1 = _i32(kind or [0]), _i32(q0 or [0]), _i32(q1 or [0])
+    ang = np.asarray(ang or [0.0], dtype=np.float64)
+    lib = load()
+    nops, nvals = ctypes.c_int(0), ctypes.c_int64(0)
+    head = (int(n), m, _ip(kind), _ip(q0), _ip(q1), _dp(ang), float(decay_factor), float(decoherence_factor),
+            float(thermal_factor), float(gate_depolarization), 1 if fused else 0,
+            FP32 if precision == 'fp32' else FP64)
+    # capacity for the largest possible program (one record per noisy gate,
+    # each with its 7-value header and 16 values, plus block headers); the
+    # call reports the sizes, and a second call follows if they do not fit
+    cap_ops = max(1, m * (n + 1 if fused else 2))
+    cap_vals = max(1, cap_ops * 30)
+    for _ in range(2):
+        ops = (DmOp * cap_ops)()
+        vals = np.empty(cap_vals)
+        rc = lib.dm_batch_circuit_program(*head, ctypes.cast(ops, ctypes.c_void_p), cap_ops, _dp(vals), cap_vals,
+                                          ctypes.byref(nops), ctypes.byref(nvals))
+        if rc == 0 or (nops.value <= cap_ops and nvals.value <= cap_vals):
+            break
+        cap_ops, cap_vals = max(1, nops.value), max(1, nvals.value)
+    check(rc, 'dm_batch_circuit_program')
+    return ops, nops.value, vals[:nvals.value]
 
 
 def cost_block_seconds(n, gate_qubits, blocks, params, precision='fp32'):

@@ -17,6 +17,19 @@ class Program:
     def __len__(self):
         return len(self._ops)
 
+    @classmethod
+    def from_compiled(cls, n, ops, nops, params):
+        """A program from ``nops`` DmOp records (a ctypes array) and their
+        values, built elsewhere (dm_batch_circuit_program)."""
+        prog = cls(n)
+        rec = np.frombuffer(ops, dtype=native.DMOP_DTYPE, count=nops)
+        prog._ops = list(zip(rec['kind'].tolist(), rec['k'].tolist(), rec['qubits'].tolist(), rec['param'].tolist()))
+        vals = np.ascontiguousarray(params, dtype=np.float64)
+        prog._vals = [vals]
+        prog._size = vals.size
+        prog._compiled = (ops, vals)
+        return prog
+
     def _add(self, kind, qubits, values=None, k=None):
         off = self._size
         if values is not None:

@@ -7,9 +7,14 @@ against the same passes in Python, on the CPU.
 * planner: identical blocks (members and qubit order) for greedy, search
   and the learned planner with the shipped cost model (both evaluate the
   network with the host forward pass), with and without lookahead;
-* cost model: the native block seconds equal CostModel.block_times.
+* cost model: the native block seconds equal CostModel.block_times;
+* batch circuit programs: the same op records (kinds, qubits, value
+  offsets, fused-record layout) as the Python builders, values to 1e-14
+  (the C library's sin / cos against numpy's, and 4x4 products summed in
+  another order).
 """
 import copy
+import ctypes
 import time
 
 import numpy as np
@@ -109,6 +114,100 @@ def test_host_forward_matches_the_oracle():
     got = native.mlp_predict_host(cost.params, costmodel.N_FEATURES, costmodel.HIDDEN, X)
     want = omlp.forward(cost.params, costmodel.N_FEATURES, costmodel.HIDDEN, X.astype(np.float64))[0]
     assert np.max(np.abs(got - want)) <= 1e-4 * max(1.0, float(np.max(np.abs(want))))
+
+
+def _same_program(a, b):
+    (ao, av), (bo, bv) = a.compile(), b.compile()
+    assert len(a) == len(b)
+    ra = np.frombuffer(ao, dtype=native.DMOP_DTYPE, count=len(a))
+    rb = np.frombuffer(bo, dtype=native.DMOP_DTYPE, count=len(b))
+    assert np.array_equal(ra, rb)
+    assert av.shape == bv.shape
+    if av.size:
+        assert np.max(np.abs(av - bv)) <= 1e-14
+    # the integer fields of fused records are exact
+    for r in ra[ra['kind'] == native.OP_FUSED]:
+        p = int(r['param'])
+        m, nq = av[p], int(av[p + 1])
+        assert m in (6.0, 7.0) and np.array_equal(av[p:p + 3 + nq], bv[p:p + 3 + nq])
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+def test_batch_circuit_program_matches_python(precision):
+    """The batching workload's circuits (widths 6-12, both forms) and one
+    circuit with u1 / u2, per-gate depolarising, dephasing and a thermal
+    feed."""
+    specs = workloads.batch(40, 3, n_range=(6, 12))
+    for n, gates, dr, _ in specs:
+        for fused in (False, True):
+            got = batch.native_program(n, gates, decay_factor=1.0 - dr, fused=fused, precision=precision)
+            want = (batch.fused_circuit_program(n, gates, decay_factor=1.0 - dr, precision=precision) if fused
+                    else batch.circuit_program(n, gates, decay_factor=1.0 - dr))
+            _same_program(got, want)
+    rng = np.random.default_rng(7)
+    n = 8
+    gates = []
+    for i in range(120):
+        r = rng.random()
+        if r < 0.2:
+            gates.append(('u1', (int(rng.integers(n)),), (float(rng.uniform(0, 6.3)),)))
+        elif r < 0.6:
+            gates.append(('u3', (int(rng.integers(n)),), tuple(float(x) for x in rng.uniform(0, 6.3, 3))))
+        else:
+            c, t = rng.choice(n, 2, replace=False)
+            gates.append(('cx', (int(c), int(t)), ()))
+    kw = dict(decay_factor=0.995, decoherence_factor=0.98, thermal_factor=0.9, gate_depolarization=1e-3)
+    _same_program(batch.native_program(n, gates, fused=False, precision=precision, **kw),
+                  batch.circuit_program(n, gates, **kw))
+    _same_program(batch.native_program(n, gates, fused=True, precision=precision, **kw),
+                  batch.fused_circuit_program(n, gates, precision=precision, **kw))
+    # u2(phi, lam) is u3(pi/2, phi, lam); identity memory noise adds no channels
+    g2 = [('u2', (1,), (0.3, 0.7)), ('cx', (0, 1), ()), ('u2', (0,), (1.1, -0.4))]
+    g3 = [('u3', (1,), (np.pi / 2, 0.3, 0.7)), ('cx', (0, 1), ()), ('u3', (0,), (np.pi / 2, 1.1, -0.4))]
+    _same_program(batch.native_program(6, g2, fused=True, precision=precision),
+                  batch.fused_circuit_program(6, g3, precision=precision))
+
+
+def test_batch_circuit_program_errors():
+    lib = native.load()
+    with pytest.raises(native.NativeError, match='n >= 6'):
+        batch.native_program(5, [('cx', (0, 1), ())], fused=True)
+    with pytest.raises(native.NativeError, match='bad qubits'):
+        batch.native_program(6, [('cx', (2, 2), ())])
+    with pytest.raises(native.NativeError, match='bad qubits'):
+        batch.native_program(6, [('u3', (6,), (0.1, 0.2, 0.3))])
+    with pytest.raises(ValueError):
+        batch.native_program(6, [('rx', (0,), (0.1,))])
+    # a kind outside u1 / u3 / cx, the size query, and too small an output
+    kind, q = native._i32([native.INST_U3, 9]), native._i32([0, 0])
+    ang = np.zeros(6)
+    nops, nv = ctypes.c_int(0), ctypes.c_int64(0)
+    rc = lib.dm_batch_circuit_program(6, 2, native._ip(kind), native._ip(q), native._ip(q), native._dp(ang), 1.0, 1.0,
+                                      1.0, 0.0, 0, native.FP64, None, 0, None, 0, ctypes.byref(nops), ctypes.byref(nv))
+    assert rc == native.ERR_ARG and 'not u1, u3 or cx' in native.last_error()
+    kind = native._i32([native.INST_U3, native.INST_CX])
+    q0, q1 = native._i32([0, 0]), native._i32([0, 1])
+    rc = lib.dm_batch_circuit_program(6, 2, native._ip(kind), native._ip(q0), native._ip(q1), native._dp(ang), 0.99,
+                                      1.0, 1.0, 0.0, 0, native.FP64, None, 0, None, 0, ctypes.byref(nops),
+                                      ctypes.byref(nv))
+    assert rc == 0 and (nops.value, nv.value) == (4, 12 + 3 + 0 + 3)
+    ops = (native.DmOp * 3)()
+    vals = np.zeros(18)
+    rc = lib.dm_batch_circuit_program(6, 2, native._ip(kind), native._ip(q0), native._ip(q1), native._dp(ang), 0.99,
+                                      1.0, 1.0, 0.0, 0, native.FP64, ctypes.cast(ops, ctypes.c_void_p), 3,
+                                      native._dp(vals), 18, ctypes.byref(nops), ctypes.byref(nv))
+    assert rc == native.ERR_ARG and 'need 4' in native.last_error()
+
+
+def test_native_batch_programs_are_fast():
+    """The 512-circuit batching workload's programs (HBM tier fused) build
+    in well under a second (the Python builders take several seconds)."""
+    specs = workloads.batch(512, 0, n_range=(6, 12))
+    native.load()
+    t0 = time.perf_counter()
+    for n, gates, dr, _ in specs:
+        batch.native_program(n, gates, decay_factor=1.0 - dr, fused=n > 7, precision='fp32')
+    assert time.perf_counter() - t0 < 1.0
 
 
 def test_native_front_end_is_fast():
