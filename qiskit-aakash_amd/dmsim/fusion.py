@@ -464,43 +464,6 @@ def tile_for(n, qs, precision='fp32'):
     return 6
 
 
-def stage_count(n, qs, sub, m):
-    """Register stages of a pass over block ``qs`` in an m-digit tile with
-    the operations ``sub`` (after compose_single_qubit_runs), by the
-    library's grouping rule (group_stages): an operation joins the first
-    stage, at or after the last stage that touches its digits, whose digit
-    set stays within two digits, else opens a new stage."""
-    inside = set(n - 1 - q for q in qs)
-    p = 0
-    while len(inside) < m and p < n:
-        inside.add(p)
-        p += 1
-    tpos = sorted(inside)
-    digit = {n - 1 - pos: j for j, pos in enumerate(tpos)}
-    stages = []
-    for g in sub:
-        D = set(digit[q] for q in g.qubits)
-        last = -1
-        for k, S in enumerate(stages):
-            if S & D:
-                last = k
-        for k in range(max(last, 0), len(stages)):
-            if len(stages[k] | D) <= 2:
-                stages[k] |= D
-                break
-        else:
-            stages.append(set(D))
-    return len(stages)
-
-
-# FP64: a block whose 6-digit tile leaves runs of 4^2 coefficients keeps 6
-# digits from this many register stages on; its 7-digit tile (128 KB of
-# LDS, one workgroup per CU) stops overlapping stages with HBM traffic
-# (measured on MI355X, 16 qubits: 1-2 stages 14.6-17.4 ms in 7 digits
-# against 18.6-19.5 ms in 6; 4 stages 23.7 against 20.9 ms)
-FP64_SHORT_RUN_STAGES = 3
-
-
 def add_block(prog, gates, qs, mem, mode='seq', tile=None, precision='fp32'):
     """Append one block to a Program.
 
@@ -523,10 +486,6 @@ def add_block(prog, gates, qs, mem, mode='seq', tile=None, precision='fp32'):
     n = prog.n
     if tile is None:
         tile = tile_for(n, qs, precision) if n >= 6 else 6
-        if (tile == 7 and precision == 'fp64' and len(qs) <= 6 and mode == 'seq'
-                and run_digits(n, [n - 1 - q for q in qs], 6) == 2
-                and stage_count(n, qs, compose_single_qubit_runs(sub), 6) >= FP64_SHORT_RUN_STAGES):
-            tile = 6
     if len(qs) == 1:
         prog.one_qubit(qs[0], block_ptm(sub, qs))
     elif (mode == 'seq' or len(qs) > 4) and n >= tile:

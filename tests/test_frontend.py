@@ -248,30 +248,3 @@ def test_shipped_cost_model_file():
     assert m.params.shape == (P,) and np.isfinite(m.params).all()
     assert m.meta['features'] == costmodel.N_FEATURES and m.meta['hidden'] == costmodel.HIDDEN
     assert m.meta['holdout_r2'] > 0.9
-
-
-def test_stage_count_follows_the_grouping_rule():
-    """fusion.stage_count groups a pass's operations as the library does:
-    an operation joins the first stage, at or after the last one touching
-    its digits, that stays within two digits.  Qubits 12..9 sit on tile
-    digits 3..6 of a 16-qubit 7-digit tile over them (positions 0-2 fill
-    digits 0-2)."""
-    from dmsim import fusion
-    import numpy as np
-    n = 16
-    R = np.eye(4)
-    g1 = lambda q: fusion.NoisyGate((q,), R, 'u3')
-    cx = lambda a, b: fusion.NoisyGate((a, b), R, 'cx')
-    qs = [9, 10, 11, 12]          # positions 6, 5, 4, 3 -> digits 6, 5, 4, 3
-    d3, d4, d5, d6 = 12, 11, 10, 9
-    # {3}, {4} join into {3, 4}; the CX on {3, 4} joins; {4, 5} opens a stage
-    assert fusion.stage_count(n, qs, [g1(d3), g1(d4), cx(d3, d4), cx(d4, d5)], 7) == 2
-    # a single-qubit op on digit 3 moves back past the {5, 6} stage
-    assert fusion.stage_count(n, qs, [cx(d3, d4), cx(d5, d6), g1(d3)], 7) == 2
-    # and past {4, 5}, which does not touch digit 3; a single-qubit op on
-    # digit 4 joins {4, 5}, the last stage touching it
-    assert fusion.stage_count(n, qs, [cx(d3, d4), cx(d4, d5), g1(d3)], 7) == 2
-    assert fusion.stage_count(n, qs, [cx(d3, d4), cx(d4, d5), g1(d4)], 7) == 2
-    # a CX on {3, 5} cannot join {4, 5} (three digits) nor move before it
-    assert fusion.stage_count(n, qs, [cx(d3, d4), cx(d4, d5), cx(d3, d5)], 7) == 3
-    assert fusion.stage_count(n, qs, [], 7) == 0
