@@ -885,6 +885,9 @@ extern "C" int dm_batch_circuit_program(int n, int ngates, const int32_t *kind, 
       max_ops < 0 || max_values < 0 || (precision != DM_FP64 && precision != DM_FP32))
     return fail(DM_ERR_ARG, "dm_batch_circuit_program: bad arguments");
   if (fused && n < 6) return fail(DM_ERR_ARG, "dm_batch_circuit_program: fused programs need n >= 6 (n=%d)", n);
+  // at most n + 1 records per gate; op counts are int
+  if ((int64_t)ngates * (n + 1) > (int64_t)INT32_MAX / 2)
+    return fail(DM_ERR_ARG, "dm_batch_circuit_program: %d gates are too many", ngates);
   const double p = gate_depolarization, lam = 1.0 - p;
   std::vector<NoisyOp> g;
   g.reserve((size_t)ngates * (fused ? n + 1 : 1));

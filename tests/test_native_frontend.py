@@ -197,6 +197,12 @@ def test_batch_circuit_program_errors():
                                       1.0, 1.0, 0.0, 0, native.FP64, ctypes.cast(ops, ctypes.c_void_p), 3,
                                       native._dp(vals), 18, ctypes.byref(nops), ctypes.byref(nv))
     assert rc == native.ERR_ARG and 'need 4' in native.last_error()
+    # a gate count whose records would overflow the op count is refused
+    # before the gate arrays are read
+    rc = lib.dm_batch_circuit_program(12, 200_000_000, native._ip(kind), native._ip(q0), native._ip(q1),
+                                      native._dp(ang), 1.0, 1.0, 1.0, 0.0, 1, native.FP64, None, 0, None, 0,
+                                      ctypes.byref(nops), ctypes.byref(nv))
+    assert rc == native.ERR_ARG and 'too many' in native.last_error()
 
 
 def test_native_batch_programs_are_fast():
