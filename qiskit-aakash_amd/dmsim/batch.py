@@ -107,6 +107,26 @@ def fused_circuit_program(n, gates, decay_factor=1.0, decoherence_factor=1.0, th
     return prog
 
 
+def concat_programs(programs):
+    """The programs' op records and values back to back, as dm_batch_create
+    takes them: (records, values, op starts, value count).  Program c's ops
+    are records [starts[c], starts[c+1]); their value offsets are shifted
+    past the values of the programs before it."""
+    recs, params, starts = [], [], [0]
+    off = 0
+    for prog in programs:
+        arr, pv = prog.compile()
+        rec = np.frombuffer(arr, dtype=native.DMOP_DTYPE, count=len(prog)).copy()
+        rec['param'] += off
+        recs.append(rec)
+        params.append(pv)
+        off += pv.size
+        starts.append(starts[-1] + len(prog))
+    ops = np.concatenate(recs) if starts[-1] else np.zeros(1, dtype=native.DMOP_DTYPE)
+    vals = np.ascontiguousarray(np.concatenate(params) if off else np.zeros(1))
+    return ops, vals, np.asarray(starts, dtype=np.int64), off
+
+
 class Batch:
     """A compiled batch on the device."""
 
@@ -114,19 +134,7 @@ class Batch:
         """``circuits``: list of (n, Program, meas_err)."""
         self.lib = native.load()
         self.ns = [int(c[0]) for c in circuits]
-        recs, params, starts = [], [], [0]
-        off = 0
-        for n, prog, _ in circuits:
-            arr, pv = prog.compile()
-            rec = np.frombuffer(arr, dtype=native.DMOP_DTYPE, count=len(prog)).copy()
-            rec['param'] += off
-            recs.append(rec)
-            params.append(pv)
-            off += pv.size
-            starts.append(starts[-1] + len(prog))
-        self._ops = np.concatenate(recs) if starts[-1] else np.zeros(1, dtype=native.DMOP_DTYPE)
-        self._params = np.ascontiguousarray(np.concatenate(params) if params else np.zeros(1))
-        self._starts = np.asarray(starts, dtype=np.int64)
+        self._ops, self._params, self._starts, off = concat_programs([c[1] for c in circuits])
         self._n = np.asarray(self.ns, dtype=np.int32)
         self._err = np.asarray([float(c[2]) for c in circuits], dtype=np.float64)
         h = ctypes.c_void_p()

@@ -519,7 +519,11 @@ def batch_circuit_program(n, gates, decay_factor=1.0, decoherence_factor=1.0, th
             break
         cap_ops, cap_vals = max(1, nops.value), max(1, nvals.value)
     check(rc, 'dm_batch_circuit_program')
-    return ops, nops.value, vals[:nvals.value]
+    # exact-size copies: the capacity bound is many times the usual program
+    k = nops.value
+    exact = (DmOp * max(1, k))()
+    ctypes.memmove(exact, ops, k * ctypes.sizeof(DmOp))
+    return exact, k, vals[:nvals.value].copy()
 
 
 def cost_block_seconds(n, gate_qubits, blocks, params, precision='fp32'):

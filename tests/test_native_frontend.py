@@ -205,6 +205,25 @@ def test_batch_circuit_program_errors():
     assert rc == native.ERR_ARG and 'too many' in native.last_error()
 
 
+def test_concatenated_programs_keep_each_circuits_values():
+    """The batch's op table: circuit c's records in [starts[c], starts[c+1]),
+    each record's values found at its shifted offset."""
+    specs = workloads.batch(12, 5, n_range=(4, 9))
+    progs = [batch.native_program(n, gates, decay_factor=1.0 - dr, fused=n > 6) for n, gates, dr, _ in specs]
+    progs.append(batch.Program(5))  # an empty program
+    ops, vals, starts, nvals = batch.concat_programs(progs)
+    assert starts[-1] == sum(len(p) for p in progs) and nvals == sum(p.compile()[1].size for p in progs)
+    for c, p in enumerate(progs):
+        arr, pv = p.compile()
+        own = np.frombuffer(arr, dtype=native.DMOP_DTYPE, count=len(p))
+        mine = ops[starts[c]:starts[c + 1]]
+        assert np.array_equal(mine[['kind', 'k', 'qubits']], own[['kind', 'k', 'qubits']])
+        ends = list(own['param'][1:]) + [pv.size]
+        for r, o, e in zip(mine, own, ends):
+            assert np.array_equal(vals[r['param']:r['param'] + (e - o['param'])], pv[o['param']:e])
+    assert batch.concat_programs([])[3] == 0
+
+
 def test_native_batch_programs_are_fast():
     """The 512-circuit batching workload's programs (HBM tier fused) build
     in well under a second (the Python builders take several seconds)."""
