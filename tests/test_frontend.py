@@ -239,6 +239,24 @@ def test_fused_tile_choice():
     assert fusion.tile_qubits(n, {0, 1, 2}, 'fp64') == {0, 1, 2, 13, 14, 15}
 
 
+def test_fp64_short_run_blocks_take_7_digits_at_any_stage_count():
+    """An FP64 block whose 6-digit tile would leave runs of 4^2 takes the
+    7-digit tile also when its operations need many register stages (a
+    6-digit rule for 3+ stages was measured slower with the current FP64
+    7-digit kernels and removed)."""
+    from dmsim import fusion, native
+    from dmsim.program import Program
+    n = 16
+    qs = [0, 1, 2, 3]                                  # positions 15..12: w6 = 2
+    gates = [fusion.NoisyGate(pair, None, 'cx') for pair in ((0, 1), (1, 2), (2, 3), (0, 3), (1, 3), (0, 2))]
+    for prec in ('fp64', 'fp32'):
+        prog = Program(n)
+        fusion.add_block(prog, gates, qs, list(range(len(gates))), 'seq', None, prec)
+        arr, vals = prog.compile()
+        assert len(prog) == 1 and arr[0].kind == native.OP_FUSED
+        assert vals[arr[0].param] == 7.0, prec
+
+
 def test_shipped_cost_model_file():
     """The trained parameters load without pickle and carry their training
     report (held-out fit of the log-time target)."""
