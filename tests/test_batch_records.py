@@ -1,0 +1,153 @@
+"""The batch circuit programs that the library builds on the host
+(dm_batch_circuit_program) against the CPU oracle, without a GPU.
+
+A numpy interpreter of the dm_op records (DM_OP_1Q, DM_OP_CX[_SCALED],
+DM_OP_MEMORY_NOISE and DM_OP_FUSED with its 1-qubit and CX sub-operations,
+applied in record order; the tile geometry does not change the result) runs
+the native programs of small circuits.  Their final Pauli coefficients must
+match the oracle's simulation of the same circuit with a barrier after every
+gate (refsim.run_experiment: the gate, then memory noise on every qubit, at
+every level; dm_simulator.py:1005-1177) to 1e-13, in both program forms (one
+record per gate and level, and fused tile passes) and with every noise
+option the builder takes.  This pins the builder's gate PTMs, the order of
+the per-gate depolarising channel and the memory noise, the CX pair scales,
+the folding of single-qubit runs and the fused-record layout against the
+reference's semantics rather than against the Python builders.
+"""
+import numpy as np
+import pytest
+
+from dmsim import assemble, batch, native, workloads
+from oracle import refsim
+
+
+def _apply_ptm_rows(st, n, q, rows):
+    """X, Y, Z rows of a trace-preserving 4x4 PTM on qubit q (axis q)."""
+    R = np.vstack([[1.0, 0.0, 0.0, 0.0], np.asarray(rows).reshape(3, 4)])
+    return np.moveaxis(np.tensordot(R, st, axes=([1], [q])), 0, q)
+
+
+def _apply_cx(st, n, c, t, scale):
+    st = refsim.apply_cx(st.reshape(-1), c, t, (1.0, 0.0), n)
+    if scale is not None:
+        s = np.asarray(scale).reshape(4, 4)  # [control component, target component]
+        shape = [1] * n
+        shape[c], shape[t] = 4, 4
+        s_b = s if c < t else s.T
+        st = st * s_b.reshape(shape)
+    return st
+
+
+def run_records(n, prog):
+    """Final coefficients (shape n*[4]) of |0...0> under the program."""
+    ops, vals = prog.compile()
+    st = refsim.initial_state(n).reshape(n * [4])
+    for i in range(len(prog)):
+        o = ops[i]
+        p = o.param
+        if o.kind == native.OP_1Q:
+            st = _apply_ptm_rows(st, n, o.qubits[0], vals[p:p + 12])
+        elif o.kind == native.OP_CX:
+            st = _apply_cx(st, n, o.qubits[0], o.qubits[1], None)
+        elif o.kind == native.OP_CX_SCALED:
+            st = _apply_cx(st, n, o.qubits[0], o.qubits[1], vals[p:p + 16])
+        elif o.kind == native.OP_MEMORY_NOISE:
+            st = refsim.apply_memory_noise(st.reshape(-1), n, vals[p], vals[p + 1], vals[p + 2])
+        elif o.kind == native.OP_FUSED:
+            m, nq = int(vals[p]), int(vals[p + 1])
+            assert m in (6, 7) and 1 <= nq <= 4
+            at = p + 2 + nq
+            nsub = int(vals[at])
+            at += 1
+            for _ in range(nsub):
+                kind, kq = int(vals[at]), int(vals[at + 1])
+                q = [int(x) for x in vals[at + 2:at + 2 + kq]]
+                nv = int(vals[at + 6])
+                v = vals[at + 7:at + 7 + nv]
+                if kind == native.SUB_1Q:
+                    assert kq == 1 and nv == 12
+                    st = _apply_ptm_rows(st, n, q[0], v)
+                else:
+                    assert kind == native.SUB_CX and kq == 2 and nv in (0, 16)
+                    st = _apply_cx(st, n, q[0], q[1], v if nv else None)
+                at += 7 + nv
+            assert at == (ops[i + 1].param if i + 1 < len(prog) else vals.size)
+        else:
+            raise AssertionError('unexpected op kind %d' % o.kind)
+        st = np.reshape(st, n * [4])
+    return st
+
+
+def oracle_state(n, gates, **opts):
+    qc = workloads.to_circuit(n, gates, barrier_after_each=True)
+    exp = assemble(qc).experiments[0]
+    data, _, _ = refsim.run_experiment(exp.instructions, n, dict(opts, compute_densitymatrix=False))
+    return np.asarray(data['coeffmatrix']).reshape(n * [4])
+
+
+def _mixed_gates(n, count, seed):
+    rng = np.random.default_rng(seed)
+    gates = []
+    for _ in range(count):
+        r = rng.random()
+        if r < 0.15:
+            gates.append(('u1', (int(rng.integers(n)),), (float(rng.uniform(0, 6.3)),)))
+        elif r < 0.6:
+            gates.append(('u3', (int(rng.integers(n)),), tuple(float(x) for x in rng.uniform(0, 6.3, 3))))
+        else:
+            c, t = rng.choice(n, 2, replace=False)
+            gates.append(('cx', (int(c), int(t)), ()))
+    return gates
+
+
+@pytest.mark.parametrize('fused', [False, True])
+def test_native_records_match_the_oracle_on_the_batch_workload(fused):
+    specs = workloads.batch(6, 21, n_range=(6, 8), depth_range=(30, 70))
+    for n, gates, dr, _ in specs:
+        prog = batch.native_program(n, gates, decay_factor=1.0 - dr, fused=fused)
+        got = run_records(n, prog)
+        want = oracle_state(n, gates, decay_factor=1.0 - dr)
+        assert np.max(np.abs(got - want)) <= 1e-13, (n, len(gates))
+
+
+@pytest.mark.parametrize('fused', [False, True])
+def test_native_records_match_the_oracle_with_every_noise_option(fused):
+    """Per-gate depolarising (1q rows and CX pair scales), dephasing and a
+    thermal feed on the memory noise, u1 gates; and noiseless memory (no
+    channels in the fused form)."""
+    n = 7
+    gates = _mixed_gates(n, 60, 4)
+    noise = dict(decay_factor=0.97, decoherence_factor=0.95, thermal_factor=0.8)
+    prog = batch.native_program(n, gates, gate_depolarization=2e-3, fused=fused, **noise)
+    want = oracle_state(n, gates, gate_depolarization=2e-3, **noise)
+    assert np.max(np.abs(run_records(n, prog) - want)) <= 1e-13
+    prog = batch.native_program(n, gates, fused=fused)
+    assert np.max(np.abs(run_records(n, prog) - oracle_state(n, gates))) <= 1e-13
+
+
+def test_interpreter_sees_a_planted_mistake():
+    """A record with one sub-operation's control and target swapped no longer
+    matches: the comparison is sharp enough to see a wrong CX direction."""
+    n = 6
+    gates = _mixed_gates(n, 40, 9)
+    prog = batch.native_program(n, gates, decay_factor=0.99, fused=True)
+    ops, vals = prog.compile()
+    cx_at = []  # value offsets of every fused CX sub-operation
+    for i in range(len(prog)):
+        if ops[i].kind != native.OP_FUSED:
+            continue
+        p = ops[i].param
+        at = p + 3 + int(vals[p + 1])
+        for _ in range(int(vals[at - 1])):
+            if int(vals[at]) == native.SUB_CX:
+                cx_at.append(at)
+            at += 7 + int(vals[at + 6])
+    assert cx_at
+    # the last one acts mid-circuit, on qubits no longer in |00> (on |00>
+    # both directions of a CX agree)
+    at = cx_at[-1]
+    bad = vals.copy()
+    bad[at + 2], bad[at + 3] = vals[at + 3], vals[at + 2]
+    prog._compiled = (ops, bad)
+    want = oracle_state(n, gates, decay_factor=0.99)
+    assert np.max(np.abs(run_records(n, prog) - want)) > 1e-6

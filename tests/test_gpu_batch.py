@@ -1,6 +1,6 @@
-"""Batched execution (LDS-resident small circuits + HBM circuits as one
-launch per round of passes, one hipGraph) against the CPU oracle, circuit by
-circuit: FP64 to 1e-12, FP32 to 2e-5 (probabilities)."""
+"""Batched execution (LDS-resident small circuits + HBM circuits as up to
+three launches per round of passes, one hipGraph) against the CPU oracle,
+circuit by circuit: FP64 to 1e-12, FP32 to 2e-5 (probabilities)."""
 import numpy as np
 import pytest
 
