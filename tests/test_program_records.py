@@ -1,18 +1,25 @@
-"""The batch circuit programs that the library builds on the host
-(dm_batch_circuit_program) against the CPU oracle, without a GPU.
+"""Op-record programs against the CPU oracle, without a GPU.
 
 A numpy interpreter of the dm_op records (DM_OP_1Q, DM_OP_CX[_SCALED],
-DM_OP_MEMORY_NOISE and DM_OP_FUSED with its 1-qubit and CX sub-operations,
-applied in record order; the tile geometry does not change the result) runs
-the native programs of small circuits.  Their final Pauli coefficients must
-match the oracle's simulation of the same circuit with a barrier after every
-gate (refsim.run_experiment: the gate, then memory noise on every qubit, at
-every level; dm_simulator.py:1005-1177) to 1e-13, in both program forms (one
-record per gate and level, and fused tile passes) and with every noise
-option the builder takes.  This pins the builder's gate PTMs, the order of
-the per-gate depolarising channel and the memory noise, the CX pair scales,
-the folding of single-qubit runs and the fused-record layout against the
-reference's semantics rather than against the Python builders.
+DM_OP_2Q, DM_OP_BLOCK, DM_OP_MEMORY_NOISE and DM_OP_FUSED with its 1-qubit,
+CX, 16x16 and dense sub-operations, applied in record order; the tile
+geometry does not change the result) runs the programs of small circuits.
+
+* Batch circuit programs built by the library (dm_batch_circuit_program):
+  their final Pauli coefficients must match the oracle's simulation of the
+  same circuit with a barrier after every gate (refsim.run_experiment: the
+  gate, then memory noise on every qubit, at every level;
+  dm_simulator.py:1005-1177) to 1e-13, in both program forms (one record
+  per gate and level, and fused tile passes) and with every noise option
+  the builder takes.  This pins the builder's gate PTMs, the order of the
+  per-gate depolarising channel and the memory noise, the CX pair scales,
+  the folding of single-qubit runs and the fused-record layout against the
+  reference's semantics rather than against the Python builders.
+* Fused programs of noisy gate lists (fusion.fused_program: the greedy and
+  learned planners, sequential and dense blocks, TSP-error CX as 16x16
+  PTMs): they must match the oracle's gate functions applied gate by gate
+  (rotations, CX, per-gate depolarising; basicaertools.py:69-126, 310-392)
+  to 1e-13.
 """
 import numpy as np
 import pytest
@@ -25,6 +32,16 @@ def _apply_ptm_rows(st, n, q, rows):
     """X, Y, Z rows of a trace-preserving 4x4 PTM on qubit q (axis q)."""
     R = np.vstack([[1.0, 0.0, 0.0, 0.0], np.asarray(rows).reshape(3, 4)])
     return np.moveaxis(np.tensordot(R, st, axes=([1], [q])), 0, q)
+
+
+def _apply_dense(st, n, qs, S):
+    """A 4^k x 4^k PTM on qubits qs, component index sum_j i_{qs[j]} 4^j."""
+    k = len(qs)
+    S = np.asarray(S).reshape((4,) * (2 * k))   # (out_{k-1} .. out_0, in_{k-1} .. in_0)
+    axes = [qs[j] for j in range(k - 1, -1, -1)]
+    moved = np.moveaxis(st, axes, list(range(k)))
+    out = np.tensordot(S, moved, axes=(list(range(k, 2 * k)), list(range(k))))
+    return np.moveaxis(out, list(range(k)), axes)
 
 
 def _apply_cx(st, n, c, t, scale):
@@ -51,11 +68,17 @@ def run_records(n, prog):
             st = _apply_cx(st, n, o.qubits[0], o.qubits[1], None)
         elif o.kind == native.OP_CX_SCALED:
             st = _apply_cx(st, n, o.qubits[0], o.qubits[1], vals[p:p + 16])
+        elif o.kind == native.OP_2Q:
+            # pair index 4 i_qa + i_qb
+            st = _apply_dense(st, n, [o.qubits[1], o.qubits[0]], vals[p:p + 256])
+        elif o.kind == native.OP_BLOCK:
+            k = o.k
+            st = _apply_dense(st, n, list(o.qubits[:k]), vals[p:p + 4 ** (2 * k)])
         elif o.kind == native.OP_MEMORY_NOISE:
             st = refsim.apply_memory_noise(st.reshape(-1), n, vals[p], vals[p + 1], vals[p + 2])
         elif o.kind == native.OP_FUSED:
             m, nq = int(vals[p]), int(vals[p + 1])
-            assert m in (6, 7) and 1 <= nq <= 4
+            assert m in (5, 6, 7) and 1 <= nq <= 7
             at = p + 2 + nq
             nsub = int(vals[at])
             at += 1
@@ -67,9 +90,15 @@ def run_records(n, prog):
                 if kind == native.SUB_1Q:
                     assert kq == 1 and nv == 12
                     st = _apply_ptm_rows(st, n, q[0], v)
-                else:
-                    assert kind == native.SUB_CX and kq == 2 and nv in (0, 16)
+                elif kind == native.SUB_CX:
+                    assert kq == 2 and nv in (0, 16)
                     st = _apply_cx(st, n, q[0], q[1], v if nv else None)
+                elif kind == native.SUB_2Q:
+                    assert kq == 2 and nv == 256
+                    st = _apply_dense(st, n, [q[1], q[0]], v)
+                else:
+                    assert kind == native.SUB_DENSE and nv == 4 ** (2 * kq)
+                    st = _apply_dense(st, n, q, v)
                 at += 7 + nv
             assert at == (ops[i + 1].param if i + 1 < len(prog) else vals.size)
         else:
@@ -151,3 +180,59 @@ def test_interpreter_sees_a_planted_mistake():
     prog._compiled = (ops, bad)
     want = oracle_state(n, gates, decay_factor=0.99)
     assert np.max(np.abs(run_records(n, prog) - want)) > 1e-6
+
+
+def oracle_gates(n, gates, p, tsp=(1.0, 0.0)):
+    """The oracle's gate functions, gate by gate: rotations of u1 / u3, CX
+    with the TSP error, then the per-gate depolarising channel."""
+    st = refsim.initial_state(n)
+    for name, qs, prm in gates:
+        if name == 'cx':
+            st = refsim.apply_cx(np.reshape(st, -1), qs[0], qs[1], tsp, n)
+        else:
+            for axis, ang in refsim.gate_rotations(name, prm):
+                q = qs[0]
+                st3 = np.reshape(st, (4 ** q, 4, 4 ** (n - q - 1)))
+                st = refsim.rotate_components(st3, axis, ang, (1.0, 0.0))
+        if p:
+            st = refsim.apply_gate_depolarizing(np.reshape(st, -1), n, list(qs), p)
+    return np.reshape(st, n * [4])
+
+
+def _sub_kinds(prog):
+    ops, vals = prog.compile()
+    kinds = set()
+    for i in range(len(prog)):
+        if ops[i].kind != native.OP_FUSED:
+            continue
+        p = ops[i].param
+        at = p + 3 + int(vals[p + 1])
+        for _ in range(int(vals[at - 1])):
+            kinds.add(int(vals[at]))
+            at += 7 + int(vals[at + 6])
+    return kinds
+
+
+@pytest.mark.parametrize('method,kmax,mode,tsp,want_ops,want_subs', [
+    ('greedy', 4, 'seq', (1.0, 0.0), {native.OP_FUSED}, {native.SUB_1Q, native.SUB_CX}),
+    ('greedy', 4, 'dense', (1.0, 0.0), {native.OP_BLOCK}, set()),
+    ('greedy', 2, 'dense', (1.0, 0.0), {native.OP_2Q}, set()),
+    ('learned', 4, 'seq', (1.0, 0.0), {native.OP_FUSED}, {native.SUB_1Q, native.SUB_CX}),
+    ('greedy', 4, 'seq', (0.97, 0.03), {native.OP_FUSED}, {native.SUB_1Q, native.SUB_2Q})])
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+def test_fused_programs_match_the_oracle(method, kmax, mode, tsp, want_ops, want_subs, precision):
+    """Sequential tile passes (6- and 7-digit, the learned planner's
+    tile-wide blocks), dense 3-4 qubit blocks and 2-qubit PTMs, and the
+    TSP-error CX inside tile passes."""
+    from dmsim import costmodel, fusion
+    n = 8
+    gates = workloads.local_u3_cx(n, 120, 5) + [('u1', (3,), (0.7,)), ('cx', (7, 2), ())]
+    noisy = fusion.noisy_gates(gates, 1e-3, tsp=tsp)
+    cost = None
+    if method == 'learned':
+        cost = costmodel.default_model()
+        cost.precision = precision
+    prog, blocks = fusion.fused_program(noisy, n, kmax, method, cost, mode, precision=precision)
+    assert want_ops <= set(prog.kinds()) and want_subs <= _sub_kinds(prog)
+    got = run_records(n, prog)
+    assert np.max(np.abs(got - oracle_gates(n, gates, 1e-3, tsp))) <= 1e-13
