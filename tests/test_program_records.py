@@ -20,6 +20,11 @@ geometry does not change the result) runs the programs of small circuits.
   PTMs): they must match the oracle's gate functions applied gate by gate
   (rotations, CX, per-gate depolarising; basicaertools.py:69-126, 310-392)
   to 1e-13.
+* The backend's level fusion (fusion.plan_levels: runs of u1/u3/cx levels
+  with the memory noise of every level as 1-qubit channels): the programs
+  of a whole circuit must match the oracle's run_experiment (merge,
+  partition, level loop with memory noise after every level) to 1e-12
+  with every noise option.
 """
 import numpy as np
 import pytest
@@ -55,10 +60,11 @@ def _apply_cx(st, n, c, t, scale):
     return st
 
 
-def run_records(n, prog):
-    """Final coefficients (shape n*[4]) of |0...0> under the program."""
+def run_records(n, prog, st=None):
+    """Final coefficients (shape n*[4]) of ``st`` (default |0...0>) under
+    the program."""
     ops, vals = prog.compile()
-    st = refsim.initial_state(n).reshape(n * [4])
+    st = (refsim.initial_state(n) if st is None else st).reshape(n * [4])
     for i in range(len(prog)):
         o = ops[i]
         p = o.param
@@ -236,3 +242,36 @@ def test_fused_programs_match_the_oracle(method, kmax, mode, tsp, want_ops, want
     assert want_ops <= set(prog.kinds()) and want_subs <= _sub_kinds(prog)
     got = run_records(n, prog)
     assert np.max(np.abs(got - oracle_gates(n, gates, 1e-3, tsp))) <= 1e-13
+
+
+@pytest.mark.parametrize('method', ['greedy', 'learned'])
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+def test_backend_level_fusion_matches_the_oracle(method, precision):
+    """Levels of several gates (no barriers), single-qubit runs merged, and
+    every noise option: thermal memory noise, dephasing, per-gate
+    depolarising, rotation error, TSP CX error."""
+    import types
+    from dmsim import fusion, schedule
+    n = 7
+    gates = _mixed_gates(n, 90, 12)
+    opts = {'decay_factor': 0.98, 'decoherence_factor': 0.96, 'thermal_factor': 0.85,
+            'gate_depolarization': 1e-3, 'rotation_error': {'rx': [1.0, 0.0], 'ry': [0.999, 0.01], 'rz': [0.998, -0.02]},
+            'tsp_model_error': [0.99, 0.02]}
+    backend = types.SimpleNamespace(
+        _decoherence_factor=opts['decoherence_factor'], _thermal_factor=opts['thermal_factor'],
+        _decay_factor=opts['decay_factor'], _fusion=method, _precision=precision,
+        _gate_depolarization=opts['gate_depolarization'], _rotation_error=opts['rotation_error'],
+        _tsp_model_error=opts['tsp_model_error'])
+    qc = workloads.to_circuit(n, gates)
+    insts = assemble(qc).experiments[0].instructions
+    levels, count = schedule.partition(schedule.merge_single_gates(insts, n, True), n)
+    plan = fusion.plan_levels(levels, count, n, backend)
+    st, clock = None, 0
+    while clock < count:
+        nxt, runner = plan[clock]   # every level holds only u1/u3/cx
+        st = run_records(n, runner.__self__, st)
+        clock = nxt
+    data, ocount, _ = refsim.run_experiment(insts, n, dict(opts, compute_densitymatrix=False))
+    assert ocount == count
+    want = np.asarray(data['coeffmatrix']).reshape(n * [4])
+    assert np.max(np.abs(st - want)) <= 1e-12
