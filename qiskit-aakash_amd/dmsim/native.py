@@ -479,7 +479,13 @@ def batch_circuit_program(n, gates, decay_factor=1.0, decoherence_factor=1.0, th
     memory-noise level; ``fused`` gives the tile-pass form."""
     m = len(gates)
     kind, q0, q1, ang = [], [], [], []
+    nprm = {'cx': 0, 'u3': 3, 'U': 3, 'u2': 2, 'u1': 1}
     for name, qs, prm in gates:
+        if name not in nprm:
+            raise ValueError('batch circuits hold u1/u2/u3/cx gates, got %s' % name)
+        if len(prm) != nprm[name] or len(qs) != (2 if name == 'cx' else 1):
+            raise ValueError('%s takes %d qubit(s) and %d parameter(s), got %r, %r'
+                             % (name, 2 if name == 'cx' else 1, nprm[name], qs, prm))
         q0.append(qs[0])
         if name == 'cx':
             kind.append(INST_CX)
@@ -493,11 +499,9 @@ def batch_circuit_program(n, gates, decay_factor=1.0, decoherence_factor=1.0, th
         elif name == 'u2':
             kind.append(INST_U3)
             ang += (np.pi / 2, prm[0], prm[1])
-        elif name == 'u1':
+        else:
             kind.append(INST_U1)
             ang += (prm[0], 0.0, 0.0)
-        else:
-            raise ValueError('batch circuits hold u1/u2/u3/cx gates, got %s' % name)
     kind, q0, q1 = _i32(kind or [0]), _i32(q0 or [0]), _i32(q1 or [0])
     ang = np.asarray(ang or [0.0], dtype=np.float64)
     lib = load()

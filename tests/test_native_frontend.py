@@ -178,6 +178,11 @@ def test_batch_circuit_program_errors():
         batch.native_program(6, [('u3', (6,), (0.1, 0.2, 0.3))])
     with pytest.raises(ValueError):
         batch.native_program(6, [('rx', (0,), (0.1,))])
+    # a short parameter list would shift every later gate's angles
+    with pytest.raises(ValueError, match='3 parameter'):
+        batch.native_program(6, [('u3', (0,), (0.1, 0.2)), ('u3', (1,), (0.1, 0.2, 0.3))])
+    with pytest.raises(ValueError, match='2 qubit'):
+        batch.native_program(6, [('cx', (0,), ())])
     # a kind outside u1 / u3 / cx, the size query, and too small an output
     kind, q = native._i32([native.INST_U3, 9]), native._i32([0, 0])
     ang = np.zeros(6)
