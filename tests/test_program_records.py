@@ -275,3 +275,24 @@ def test_backend_level_fusion_matches_the_oracle(method, precision):
     assert ocount == count
     want = np.asarray(data['coeffmatrix']).reshape(n * [4])
     assert np.max(np.abs(st - want)) <= 1e-12
+
+
+def test_native_records_over_random_configurations():
+    """24 seeded configurations: widths 6-9, depths 10-60, random memory
+    noise (decay, dephasing, thermal feed or none), per-gate depolarising on
+    or off, both forms and both planning precisions."""
+    rng = np.random.default_rng(2024)
+    for trial in range(24):
+        n = int(rng.integers(6, 10))
+        gates = _mixed_gates(n, int(rng.integers(10, 61)), 1000 + trial)
+        if rng.random() < 0.25:
+            noise = {}
+        else:
+            noise = dict(decay_factor=float(rng.uniform(0.95, 1.0)), decoherence_factor=float(rng.uniform(0.95, 1.0)),
+                         thermal_factor=float(rng.choice([1.0, rng.uniform(0.5, 1.0)])))
+        gd = float(rng.choice([0.0, rng.uniform(1e-4, 1e-2)]))
+        fused = bool(rng.integers(2))
+        precision = 'fp32' if rng.integers(2) else 'fp64'
+        prog = batch.native_program(n, gates, gate_depolarization=gd, fused=fused, precision=precision, **noise)
+        want = oracle_state(n, gates, gate_depolarization=gd, **noise)
+        assert np.max(np.abs(run_records(n, prog) - want)) <= 1e-13, (trial, n, fused, precision, noise, gd)
