@@ -667,6 +667,9 @@ int dm_plan_fusion(int n, int ngates, const int32_t *gate_qubits, int method, in
     g[i].q[0] = a;
     g[i].q[1] = b < 0 ? a : b;
     g[i].mask = (1u << a) | (b < 0 ? 0u : 1u << b);
+    // a block of at most kmax qubits must be able to hold every gate, or the
+    // plan could not advance past it
+    if (g[i].nq > kmax) return fail(DM_ERR_ARG, "dm_plan_fusion: kmax=%d cannot hold gate %d on 2 qubits", kmax, i);
   }
   const bool allow_mfma = n >= kMinTileQubits;
   const int km = allow_mfma ? kmax : std::min(kmax, 2);
@@ -736,6 +739,9 @@ int dm_plan_fusion(int n, int ngates, const int32_t *gate_qubits, int method, in
         pick = cands[choice];
       }
     }
+    // every block takes at least one pending gate, so there are at most
+    // ngates blocks; an empty one would never advance
+    if (pick.members.empty()) return fail(DM_ERR_ARG, "dm_plan_fusion: no block takes gate %d", first);
     for (int i : pick.members) {
       done[i] = 1;
       out_members[nm++] = i;
@@ -1021,7 +1027,7 @@ extern "C" int dm_batch_circuit_program(int n, int ngates, const int32_t *kind, 
   if (!ops || !values || *nops > max_ops || *nvalues > max_values)
     return fail(DM_ERR_ARG, "dm_batch_circuit_program: output holds %d ops / %lld values, need %d / %lld", max_ops,
                 (long long)max_values, *nops, (long long)*nvalues);
-  memcpy(ops, out.ops.data(), out.ops.size() * sizeof(dm_op));
-  memcpy(values, out.vals.data(), out.vals.size() * sizeof(double));
+  if (!out.ops.empty()) memcpy(ops, out.ops.data(), out.ops.size() * sizeof(dm_op));
+  if (!out.vals.empty()) memcpy(values, out.vals.data(), out.vals.size() * sizeof(double));
   return DM_OK;
 }

@@ -1,0 +1,160 @@
+"""The host front end (csrc/frontend.cpp: merge, partition, fusion planner,
+batch circuit programs) under AddressSanitizer and UndefinedBehaviorSanitizer,
+on the CPU.
+
+frontend.cpp is plain host C++ (no device code), so it is compiled here with
+g++ together with a driver that feeds each entry point seeded random inputs,
+valid and invalid, with output arrays of exactly the sizes include/dmsim.h
+states (and a too-small one for the batch builder).  Any out-of-bounds
+access, use of uninitialised memory through an invalid index, leak or
+undefined behaviour ends the run with a sanitizer report; the driver also
+checks that every call returns 0 or DM_ERR_ARG.  The error-message hook that
+the library defines in its device source file is supplied by the driver.
+"""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+DRIVER = r'''
+#include <stdio.h>
+#include <string.h>
+#include <random>
+#include <vector>
+#include "dmsim.h"
+
+namespace dmsim_internal {
+int set_error(int code, const char *) { return code; }
+}
+
+static std::mt19937_64 rng(12345);
+static int uni(int lo, int hi) { return std::uniform_int_distribution<int>(lo, hi)(rng); }
+static double unif(double lo, double hi) { return std::uniform_real_distribution<double>(lo, hi)(rng); }
+static int fails = 0;
+static void expect_ok_or_arg(int rc, const char *what) {
+  if (rc != 0 && rc != DM_ERR_ARG) { fprintf(stderr, "%s returned %d\n", what, rc); ++fails; }
+}
+
+static void batch_programs() {
+  for (int trial = 0; trial < 300; ++trial) {
+    const int n = uni(1, 12), m = uni(0, 120);
+    std::vector<int32_t> kind(m + 1), q0(m + 1), q1(m + 1);
+    std::vector<double> ang(3 * (m + 1));
+    for (int i = 0; i < m; ++i) {
+      const int r = uni(0, 99);
+      kind[i] = r < 20 ? DM_INST_U1 : r < 60 ? DM_INST_U3 : r < 99 ? DM_INST_CX : uni(-3, 12);
+      q0[i] = uni(0, n - 1);
+      q1[i] = uni(0, n - 1);
+      if (uni(0, 200) == 0) q0[i] = uni(-2, n + 2);
+      for (int j = 0; j < 3; ++j) ang[3 * i + j] = unif(-7, 7);
+    }
+    const double decay = uni(0, 3) ? unif(0.9, 1.0) : 1.0, deco = uni(0, 1) ? unif(0.9, 1.0) : 1.0;
+    const double therm = unif(0.0, 1.0), gd = uni(0, 1) ? unif(0.0, 0.05) : 0.0;
+    const int fused = uni(0, 1), prec = uni(0, 1) ? DM_FP32 : DM_FP64;
+    int nops = -1;
+    int64_t nv = -1;
+    int rc = dm_batch_circuit_program(n, m, kind.data(), q0.data(), q1.data(), ang.data(), decay, deco, therm, gd,
+                                      fused, prec, nullptr, 0, nullptr, 0, &nops, &nv);
+    expect_ok_or_arg(rc, "size query");
+    if (rc) continue;
+    std::vector<dm_op> ops(nops > 0 ? nops : 1);
+    std::vector<double> vals(nv > 0 ? nv : 1);
+    int nops2 = -1;
+    int64_t nv2 = -1;
+    rc = dm_batch_circuit_program(n, m, kind.data(), q0.data(), q1.data(), ang.data(), decay, deco, therm, gd, fused,
+                                  prec, ops.data(), nops, vals.data(), nv, &nops2, &nv2);
+    if (rc != 0 || nops2 != nops || nv2 != nv) { fprintf(stderr, "fill mismatch\n"); ++fails; }
+    for (int i = 0; i < nops; ++i)
+      if (ops[i].param < 0 || ops[i].param > nv) { fprintf(stderr, "bad offset\n"); ++fails; }
+    if (nops > 0) {  // one record short
+      rc = dm_batch_circuit_program(n, m, kind.data(), q0.data(), q1.data(), ang.data(), decay, deco, therm, gd,
+                                    fused, prec, ops.data(), nops - 1, vals.data(), nv, &nops2, &nv2);
+      if (rc != DM_ERR_ARG) { fprintf(stderr, "short output accepted\n"); ++fails; }
+    }
+  }
+}
+
+static void plans() {
+  const int F = 34, H = 64;
+  std::vector<float> mlp((size_t)F * H + H + (size_t)H * H + H + H + 1);
+  for (auto &x : mlp) x = (float)unif(-0.3, 0.3);
+  for (int trial = 0; trial < 120; ++trial) {
+    const int n = uni(2, 16), ng = uni(0, 150);
+    std::vector<int32_t> gq(2 * ng + 2);
+    for (int i = 0; i < ng; ++i) {
+      gq[2 * i] = uni(0, n - 1);
+      gq[2 * i + 1] = uni(0, 2) ? -1 : (gq[2 * i] + uni(1, n - 1)) % n;
+    }
+    const int method = uni(0, 2), kmax = uni(1, 4), tiles = uni(0, 1);
+    int32_t la[2] = {uni(1, 3), uni(1, 2)};
+    const int nla = method == DM_PLAN_LEARNED ? uni(0, 2) : 0;
+    std::vector<int32_t> members(ng + 1), bstart(ng + 1), qubits(2 * ng + 1), qstart(ng + 1);
+    int nb = -1;
+    const int rc = dm_plan_fusion(n, ng, gq.data(), method, kmax, tiles, la, nla, mlp.data(), uni(0, 1) ? DM_FP32 : DM_FP64,
+                                  members.data(), bstart.data(), qubits.data(), qstart.data(), &nb);
+    expect_ok_or_arg(rc, "dm_plan_fusion");
+    if (rc == 0 && (nb < 0 || (ng > 0 && bstart[nb] != ng))) { fprintf(stderr, "plan lost gates\n"); ++fails; }
+  }
+}
+
+static void partitions_and_merges() {
+  for (int trial = 0; trial < 200; ++trial) {
+    const int n = uni(1, 8), ni = uni(0, 80);
+    std::vector<int32_t> kind(ni + 1), q0(ni + 1), q1(ni + 1);
+    std::vector<double> prm(3 * (ni + 1));
+    for (int i = 0; i < ni; ++i) {
+      kind[i] = uni(DM_INST_ROT, DM_INST_OWN_LEVEL);
+      q0[i] = uni(0, n - 1);
+      q1[i] = n > 1 ? (q0[i] + uni(1, n - 1)) % n : 0;
+      if (kind[i] == DM_INST_CX && n < 2) kind[i] = DM_INST_ROT;
+    }
+    std::vector<int32_t> ops(ni + 1), starts(4 * ni + 4);
+    int nl = -1, cnt = -1;
+    expect_ok_or_arg(dm_partition(n, ni, kind.data(), q0.data(), q1.data(), ops.data(), starts.data(), &nl, &cnt),
+                     "dm_partition");
+    for (int i = 0; i < ni; ++i) {
+      kind[i] = uni(DM_INST_U1, DM_INST_SEPARATOR);
+      for (int j = 0; j < 3; ++j) prm[3 * i + j] = unif(-7, 7);
+    }
+    std::vector<int32_t> src(ni + 1), okind(ni + 1);
+    std::vector<double> oprm(3 * (ni + 1));
+    int nout = -1;
+    expect_ok_or_arg(dm_merge_rotations(ni, kind.data(), q0.data(), prm.data(), uni(0, 1), src.data(), okind.data(),
+                                        oprm.data(), &nout), "dm_merge_rotations");
+  }
+}
+
+int main() {
+  batch_programs();
+  plans();
+  partitions_and_merges();
+  printf("fails %d\n", fails);
+  return fails ? 1 : 0;
+}
+'''
+
+
+@pytest.fixture(scope='module')
+def sanitized_driver(tmp_path_factory):
+    if shutil.which('g++') is None:
+        pytest.skip('no host C++ compiler')
+    d = tmp_path_factory.mktemp('frontend_asan')
+    drv = d / 'driver.cpp'
+    drv.write_text(DRIVER)
+    exe = d / 'driver'
+    subprocess.run(['g++', '-std=c++17', '-O1', '-g', '-fno-omit-frame-pointer', '-fsanitize=address,undefined',
+                    '-fno-sanitize-recover=all', '-I', os.path.join(ROOT, 'include'),
+                    os.path.join(ROOT, 'qiskit-aakash_amd', 'csrc', 'frontend.cpp'), str(drv), '-o', str(exe)],
+                   check=True, timeout=300)
+    return exe
+
+
+def test_front_end_is_clean_under_sanitizers(sanitized_driver):
+    env = dict(os.environ, ASAN_OPTIONS='verify_asan_link_order=0:detect_leaks=1:abort_on_error=0',
+               UBSAN_OPTIONS='print_stacktrace=1:halt_on_error=1')
+    res = subprocess.run([str(sanitized_driver)], capture_output=True, text=True, timeout=300, env=env)
+    assert res.returncode == 0, res.stderr[-4000:] + res.stdout[-500:]
+    assert 'fails 0' in res.stdout

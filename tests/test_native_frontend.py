@@ -96,6 +96,16 @@ def test_learned_plan_matches_python(lookahead, precision):
             assert a == b, (n, kind, tiles)
 
 
+def test_plan_refuses_blocks_too_small_for_a_gate():
+    """kmax = 1 cannot hold a CX: the planner refuses instead of emitting
+    empty blocks without end (which overran its output arrays)."""
+    gates = _gates(6, 20, 3, 'local')
+    with pytest.raises(native.NativeError, match='kmax=1 cannot hold'):
+        fusion.plan(gates, 6, 1, 'greedy', tiles=False)
+    ones = [g for g in gates if len(g.qubits) == 1]
+    assert fusion.plan(ones, 6, 1, 'greedy', tiles=False) == fusion.plan_py(ones, 6, 1, 'greedy', tiles=False)
+
+
 def test_cost_block_seconds_match_python():
     cost = costmodel.CostModel.load(precision='fp32')
     n = 14
