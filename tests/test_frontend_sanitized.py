@@ -1,6 +1,6 @@
 """The host front end (csrc/frontend.cpp: merge, partition, fusion planner,
-batch circuit programs) under AddressSanitizer and UndefinedBehaviorSanitizer,
-on the CPU.
+its cost model on the host, batch circuit programs) under AddressSanitizer
+and UndefinedBehaviorSanitizer, on the CPU.
 
 frontend.cpp is plain host C++ (no device code), so it is compiled here with
 g++ together with a driver that feeds each entry point seeded random inputs,
@@ -97,7 +97,16 @@ static void plans() {
                                   members.data(), bstart.data(), qubits.data(), qstart.data(), &nb);
     expect_ok_or_arg(rc, "dm_plan_fusion");
     if (rc == 0 && (nb < 0 || (ng > 0 && bstart[nb] != ng))) { fprintf(stderr, "plan lost gates\n"); ++fails; }
+    if (rc == 0 && nb > 0) {  // the cost model's seconds of the plan's blocks
+      std::vector<double> secs(nb);
+      expect_ok_or_arg(dm_cost_block_seconds(n, ng, gq.data(), nb, members.data(), bstart.data(), qubits.data(),
+                                             qstart.data(), mlp.data(), DM_FP32, secs.data()),
+                       "dm_cost_block_seconds");
+    }
   }
+  std::vector<float> X(7 * F), y(7);
+  for (auto &x : X) x = (float)unif(-1, 1);
+  expect_ok_or_arg(dm_mlp_predict_host(mlp.data(), F, H, X.data(), 7, y.data()), "dm_mlp_predict_host");
 }
 
 static void partitions_and_merges() {
