@@ -82,7 +82,7 @@ static void plans() {
   std::vector<float> mlp((size_t)F * H + H + (size_t)H * H + H + H + 1);
   for (auto &x : mlp) x = (float)unif(-0.3, 0.3);
   for (int trial = 0; trial < 120; ++trial) {
-    const int n = uni(2, 16), ng = uni(0, 150);
+    const int n = uni(2, 20), ng = uni(0, 150);
     std::vector<int32_t> gq(2 * ng + 2);
     for (int i = 0; i < ng; ++i) {
       gq[2 * i] = uni(0, n - 1);
