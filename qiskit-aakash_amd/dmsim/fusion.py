@@ -355,6 +355,8 @@ def plan(gates, n, kmax=4, method='greedy', cost=None, tiles=True, lookahead=LOO
 def plan_py(gates, n, kmax=4, method='greedy', cost=None, tiles=True, lookahead=LOOKAHEAD):
     """The planner in Python (the cross-check of the native one): blocks as
     (qubits, [gate indices]) in execution order."""
+    if any(len(g.qubits) > kmax for g in gates):
+        raise ValueError('kmax=%d cannot hold a gate on 2 qubits' % kmax)
     done = [False] * len(gates)
     blocks = []
     first = 0

@@ -102,6 +102,8 @@ def test_plan_refuses_blocks_too_small_for_a_gate():
     gates = _gates(6, 20, 3, 'local')
     with pytest.raises(native.NativeError, match='kmax=1 cannot hold'):
         fusion.plan(gates, 6, 1, 'greedy', tiles=False)
+    with pytest.raises(ValueError, match='kmax=1 cannot hold'):
+        fusion.plan_py(gates, 6, 1, 'greedy', tiles=False)
     ones = [g for g in gates if len(g.qubits) == 1]
     assert fusion.plan(ones, 6, 1, 'greedy', tiles=False) == fusion.plan_py(ones, 6, 1, 'greedy', tiles=False)
 
